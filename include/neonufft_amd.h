@@ -1,0 +1,167 @@
+/*
+ * neonufft_amd.h -- C ABI of the MI355X (gfx950) NeoNUFFT GPU plans.
+ *
+ * Every entry point is a plain-C mirror of one member of the reference's GPU
+ * plan classes neonufft::gpu::Plan<T,DIM> / neonufft::gpu::PlanT3<T,DIM>
+ * (include/neonufft/gpu/plan.hpp of the reference; the C++ classes of this
+ * library, include/neonufft/gpu/plan.hpp, are built on top of the same code).
+ * A binding (ctypes, cffi, JNI, cgo) needs nothing but this header.
+ *
+ * Conventions:
+ *  - precision: NNF_F32 (T = float, complex = 2 x float interleaved) or
+ *    NNF_F64 (T = double). dim: 1, 2 or 3.
+ *  - Every data pointer is a DEVICE pointer. Coordinate arrays are passed as
+ *    an array of `dim` pointers to contiguous T arrays (struct of arrays),
+ *    as in the reference's std::array<const T*, DIM>.
+ *  - Mode arrays are column-major (x fastest) with element strides.
+ *  - `stream` is a hipStream_t (NULL = the null stream). Work is enqueued on it.
+ *    Plan creation and set_points synchronise the stream; transforms do not.
+ *  - Return value: a NeonufftError code (include/neonufft/errors.h, same values
+ *    as the reference). On failure nnf_last_error() returns a message for the
+ *    calling thread.
+ *  - A plan must not be used from two host threads at once (it owns its grids).
+ */
+#ifndef NEONUFFT_AMD_H
+#define NEONUFFT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NNF_F32 0
+#define NNF_F64 1
+
+/* Mirrors neonufft::Options (include/neonufft/plan.hpp:19-29 of the reference),
+ * with the C++ bools as ints. order: 0 = NEONUFFT_MODE_ORDER_FFT,
+ * 1 = NEONUFFT_MODE_ORDER_CMCL (default). */
+typedef struct nnf_options {
+  double tol;
+  double upsampfac;
+  double recenter_threshold;
+  int num_threads;
+  int sort_input;
+  int sort_output;
+  int kernel_approximation;
+  int order;
+  int kernel_type;
+} nnf_options;
+
+typedef struct nnf_plan_s* nnf_plan;
+typedef struct nnf_plan_t3_s* nnf_plan_t3;
+
+/* Fills the reference defaults: tol 1e-3, upsampfac 2, recenter 0.1, CMCL. */
+void nnf_options_default(nnf_options* opt);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* nnf_last_error(void);
+
+/* Library version string. */
+const char* nnf_version(void);
+
+/* ---- Type 1 / type 2 plan --------------------------------------------- */
+
+/* gpu::Plan<T,DIM>::Plan(opt, sign, num_nu, loc, modes, stream)
+ * (include/neonufft/gpu/plan.hpp:30-32). Folds, rescales and bins the points
+ * (as set_points) and synchronises the stream. */
+int nnf_plan_create(nnf_plan* plan, int precision, int dim, const nnf_options* opt, int sign,
+                    int64_t num_nu, const void* const* loc, const int64_t* modes, void* stream);
+
+/* gpu::Plan<T,DIM>::set_points (include/neonufft/gpu/plan.hpp:42). */
+int nnf_plan_set_points(nnf_plan plan, int64_t num_nu, const void* const* loc);
+
+/* gpu::Plan<T,DIM>::transform_type_1 (include/neonufft/gpu/plan.hpp:34-35):
+ * in = num_nu complex strengths, out = modes written with out_strides. */
+int nnf_plan_transform_type_1(nnf_plan plan, const void* in, void* out,
+                              const int64_t* out_strides);
+
+/* gpu::Plan<T,DIM>::transform_type_2 (include/neonufft/gpu/plan.hpp:37-39):
+ * in = modes read with in_strides, out = num_nu complex values. */
+int nnf_plan_transform_type_2(nnf_plan plan, const void* in, const int64_t* in_strides,
+                              void* out);
+
+/* Destroys the plan (the reference's pImpl deleter, src/neonufft/gpu/plan.cpp:17-21).
+ * Does not synchronise: the caller must not destroy a plan with work in flight. */
+int nnf_plan_destroy(nnf_plan plan);
+
+/* ---- Type 3 plan --------------------------------------------------------- */
+
+/* gpu::PlanT3<T,DIM>::grid_memory_size (include/neonufft/gpu/plan.hpp:53-57).
+ * Bounds are given as doubles (exact for float plans). */
+int nnf_plan_t3_grid_memory_size(int precision, int dim, const nnf_options* opt,
+                                 const double* input_min, const double* input_max,
+                                 const double* output_min, const double* output_max,
+                                 uint64_t* size);
+
+/* gpu::PlanT3<T,DIM>::PlanT3(opt, sign, num_in, input_points, num_out,
+ * output_points, stream, batch_size) (include/neonufft/gpu/plan.hpp:59-62). */
+int nnf_plan_t3_create(nnf_plan_t3* plan, int precision, int dim, const nnf_options* opt,
+                       int sign, int64_t num_in, const void* const* input_points, int64_t num_out,
+                       const void* const* output_points, void* stream, int64_t batch_size);
+
+/* gpu::PlanT3<T,DIM>::PlanT3(opt, sign, input_min, input_max, output_min,
+ * output_max, stream, batch_size) (include/neonufft/gpu/plan.hpp:64-67). */
+int nnf_plan_t3_create_bounds(nnf_plan_t3* plan, int precision, int dim, const nnf_options* opt,
+                              int sign, const double* input_min, const double* input_max,
+                              const double* output_min, const double* output_max, void* stream,
+                              int64_t batch_size);
+
+/* gpu::PlanT3<T,DIM>::set_input_points (include/neonufft/gpu/plan.hpp:69-70). */
+int nnf_plan_t3_set_input_points(nnf_plan_t3 plan, int64_t num_in, const void* const* points);
+
+/* gpu::PlanT3<T,DIM>::set_output_points (include/neonufft/gpu/plan.hpp:72-73). */
+int nnf_plan_t3_set_output_points(nnf_plan_t3 plan, int64_t num_out, const void* const* points);
+
+/* gpu::PlanT3<T,DIM>::add_input (include/neonufft/gpu/plan.hpp:75); bdist <= 0 means num_in. */
+int nnf_plan_t3_add_input(nnf_plan_t3 plan, const void* in, int64_t bdist);
+
+/* gpu::PlanT3<T,DIM>::transform (include/neonufft/gpu/plan.hpp:77); bdist <= 0 means num_out. */
+int nnf_plan_t3_transform(nnf_plan_t3 plan, void* out, int64_t bdist);
+
+/* gpu::PlanT3<T,DIM>::reset (include/neonufft/gpu/plan.hpp:79). */
+int nnf_plan_t3_reset(nnf_plan_t3 plan);
+
+int nnf_plan_t3_destroy(nnf_plan_t3 plan);
+
+/* ---- Stage-level extension (no reference counterpart) --------------------
+ * The reference runs these stages only inside transform_type_{1,2}
+ * (src/neonufft/gpu/plan_impl.hpp:91-136). They are exposed for stage-level
+ * parity tests and for timing the spreading / interpolation kernels alone. */
+
+typedef struct nnf_plan_info {
+  int precision;
+  int dim;
+  int n_spread;          /* kernel width ns */
+  double es_beta;        /* ES kernel shape */
+  int64_t modes[3];      /* N_d (type 1/2) or spread-grid size (type 3) */
+  int64_t grid[3];       /* fine (FFT) grid n_d */
+  int64_t spread_grid[3];/* grid the points are spread onto (type 3: n_d, else = grid) */
+  int64_t bin_size[3];   /* subgrid tile (bin) edge per dim */
+  int64_t num_bins;
+  int64_t num_subproblems;
+  int64_t num_points;
+  int64_t num_out_points;/* type 3 targets */
+  int horner_degree;     /* fp32 ES polynomial degree (0 = exact exp/sqrt) */
+  int quad_nodes;        /* Gauss-Legendre nodes on (0,1) for phihat */
+} nnf_plan_info;
+
+int nnf_plan_get_info(nnf_plan plan, nnf_plan_info* info);
+int nnf_plan_t3_get_info(nnf_plan_t3 plan, nnf_plan_info* info);
+
+/* Zero the fine grid and spread `in` onto it (type-1 stage 1). */
+int nnf_plan_spread(nnf_plan plan, const void* in);
+/* Interpolate the current fine grid at the plan's points into `out`
+ * (type-2 stage 3). */
+int nnf_plan_interp(nnf_plan plan, void* out);
+/* Device pointer of the plan's fine grid (n_0 x n_1 x n_2 complex, x fastest). */
+int nnf_plan_fine_grid(nnf_plan plan, void** grid);
+/* Device pointer of the deconvolution factors fs_d[0..n_d/2] (T). */
+int nnf_plan_fseries(nnf_plan plan, int d, const void** fs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEONUFFT_AMD_H */

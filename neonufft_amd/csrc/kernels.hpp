@@ -1,0 +1,97 @@
+// Host-side launchers of the HIP kernels (defined in the *.hip files with
+// explicit instantiations for T in {float, double}, DIM in {1, 2, 3}).
+#pragma once
+
+#include "common.hpp"
+
+namespace nnf {
+
+template <typename T>
+struct Loc {
+  const T* p[3];
+};
+
+// Quadrature for the Fourier transform of the ES kernel:
+// phihat(f) = sum_q weight[q] * cos(f * node[q]), with node = z * ns/2 and
+// weight = ns * w * phi(node) for the positive half (z, w) of a 2*nq point
+// Gauss-Legendre rule (phi is even). Built on the host in FP64.
+constexpr int kMaxQuad = 32;
+struct QuadRule {
+  double node[kMaxQuad];
+  double weight[kMaxQuad];
+  int nq;
+};
+
+// ---- points.hip: fold / rescale, counting sort by bin, subproblems -------
+template <typename T, int DIM>
+void launch_bin_count(Loc<T> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
+                      uint32_t* count, uint32_t* rank, hipStream_t stream);
+
+template <typename T, int DIM>
+void launch_bin_scatter(Loc<T> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
+                        const uint32_t* bin_start, const uint32_t* rank, PointRec<T, DIM>* pts,
+                        uint32_t* perm, hipStream_t stream);
+
+// Exclusive prefix sum of n uint32 values; `scratch` must hold
+// scan_scratch_size(n) values.
+std::size_t scan_scratch_size(std::size_t n);
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, std::size_t n, uint32_t* scratch,
+                           hipStream_t stream);
+
+void launch_subprob_counts(const uint32_t* count, std::size_t nbins, uint32_t max_points,
+                           uint32_t* nsub, hipStream_t stream);
+void launch_subprob_build(const uint32_t* count, const uint32_t* bin_start,
+                          const uint32_t* sub_start, std::size_t nbins, uint32_t max_points,
+                          Subprob* subs, hipStream_t stream);
+
+// Per-block partial min/max of x (as double): partial[2*b] = min, [2*b+1] = max.
+int minmax_blocks(std::size_t m);
+template <typename T>
+void launch_minmax(const T* x, std::size_t m, double* partial, hipStream_t stream);
+
+// ---- spread_interp.hip: ES spreading / interpolation on binned points -----
+template <typename T, int DIM>
+void launch_spread(int ns, const PointRec<T, DIM>* pts, const uint32_t* perm, const Subprob* subs,
+                   std::size_t nsub, const Cplx<T>* in, const Cplx<T>* pre_sorted, Cplx<T>* grid,
+                   const BinGeom& g, const EsParams& es, hipStream_t stream);
+
+template <typename T, int DIM>
+void launch_interp(int ns, const PointRec<T, DIM>* pts, const uint32_t* perm, const Subprob* subs,
+                   std::size_t nsub, const Cplx<T>* grid, const Cplx<T>* post_sorted, Cplx<T>* out,
+                   const BinGeom& g, const EsParams& es, hipStream_t stream);
+
+// Dynamic LDS bytes of one spreading / interpolation block for a bin geometry.
+template <typename T>
+std::size_t tile_bytes(const BinGeom& g);
+
+// ---- grid.hip: deconvolution factors, up/down-sampling, phases -----------
+template <typename T>
+void launch_fill_zero(Cplx<T>* p, std::size_t count, hipStream_t stream);
+
+// fs[k] = (-1)^k / phihat(2 pi k / n), k = 0..n/2.
+template <typename T>
+void launch_fseries(T* fs, IntType n, const QuadRule& rule, hipStream_t stream);
+
+// order_fft: 1 = NEONUFFT_MODE_ORDER_FFT, 0 = CMCL. Strides in elements.
+template <typename T, int DIM>
+void launch_upsample(int order_fft, const Cplx<T>* in, const IntType* in_strides,
+                     const IntType* modes, T* const* fs, Cplx<T>* grid, const IntType* n,
+                     hipStream_t stream);
+
+template <typename T, int DIM>
+void launch_downsample(int order_fft, const Cplx<T>* grid, const IntType* n, T* const* fs,
+                       Cplx<T>* out, const IntType* out_strides, const IntType* modes,
+                       hipStream_t stream);
+
+// Type-3 phase factors, written in bin (sorted) order: value[p] belongs to
+// point perm[p].
+template <typename T, int DIM>
+void launch_prephase(int sign, Loc<T> loc, const uint32_t* perm, std::size_t m,
+                     const double* sbar, Cplx<T>* pre_sorted, hipStream_t stream);
+
+template <typename T, int DIM>
+void launch_postphase(int sign, Loc<T> loc, const uint32_t* perm, std::size_t m,
+                      const double* xbar, const double* sbar, const double* scale,
+                      const QuadRule& rule, Cplx<T>* post_sorted, hipStream_t stream);
+
+}  // namespace nnf

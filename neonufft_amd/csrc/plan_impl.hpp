@@ -1,0 +1,361 @@
+// Plan orchestration: the type 1/2 pipeline and the type-3 pipeline.
+//
+// Mirrors the stage sequence of the reference's gpu::PlanImpl and
+// gpu::PlanT3Impl (src/neonufft/gpu/plan_impl.hpp:36-167,
+// src/neonufft/gpu/plan_t3_impl.hpp:48-326): fold + bin on set_points,
+// spread -> FFT -> deconvolve for type 1, deconvolve -> FFT -> interpolate
+// for type 2, and prephase x spread, deconvolve -> FFT -> interpolate x
+// postphase for type 3. All work is enqueued on the plan's stream.
+#pragma once
+
+#include <neonufft/gpu/plan.hpp>
+#include <neonufft/plan.hpp>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "../../include/neonufft_amd.h"
+#include "fft.hpp"
+#include "params.hpp"
+#include "point_set.hpp"
+
+namespace nnf {
+
+struct PlanTuning {
+  int bin[3] = {0, 0, 0};  // 0 = default
+  uint32_t max_points = kDefaultMaxPoints;
+};
+
+template <typename T, int DIM>
+class PlanImpl {
+public:
+  PlanImpl(const neonufft::Options& opt, int sign, IntType num_nu, const T* const* loc,
+           const IntType* modes, hipStream_t stream, std::shared_ptr<neonufft::Allocator> alloc,
+           const PlanTuning& tune = PlanTuning())
+      : opt_(opt), sign_(sign), stream_(stream), alloc_(std::move(alloc)), tune_(tune) {
+    if (!alloc_) throw neonufft::InputError("Neonufft:: null allocator");
+    ks_ = make_kernel_setup(opt_.tol, opt_.upsampfac, std::is_same_v<T, float>);
+    for (int d = 0; d < 3; ++d) {
+      modes_[d] = 1;
+      n_[d] = 1;
+    }
+    std::size_t cells = 1;
+    for (int d = 0; d < DIM; ++d) {
+      if (modes[d] <= 0) throw neonufft::InputError("Neonufft:: modes must be positive");
+      modes_[d] = modes[d];
+      // n_d = max(2 ns, N_d * sigma) truncated (gpu/plan_impl.hpp:56-60)
+      n_[d] = std::max<IntType>(2 * ks_.ns,
+                                static_cast<IntType>(static_cast<double>(modes[d]) * opt_.upsampfac));
+      cells *= static_cast<std::size_t>(n_[d]);
+    }
+    grid_.reset(cells, alloc_);
+    fft_ = FftPlan(std::is_same_v<T, double>, DIM, n_, sign_, 1, stream_, alloc_);
+    for (int d = 0; d < DIM; ++d) {
+      fs_[d].reset(static_cast<std::size_t>(n_[d] / 2 + 1), alloc_);
+      launch_fseries<T>(fs_[d].data(), n_[d], ks_.quad, stream_);
+    }
+    geom_ = choose_bins(DIM, n_, ks_.ns, sizeof(Cplx<T>), tune_.bin);
+    set_points(num_nu, loc);
+  }
+
+  void set_points(IntType num_nu, const T* const* loc) {
+    if (num_nu < 0) throw neonufft::InputError("Neonufft:: negative number of points");
+    Loc<T> l{{nullptr, nullptr, nullptr}};
+    for (int d = 0; d < DIM; ++d) l.p[d] = loc ? loc[d] : nullptr;
+    FoldParams fp{{0, 0, 0}, {1, 1, 1}};
+    pts_.build(l, static_cast<std::size_t>(num_nu), fp, geom_, ks_.ns, tune_.max_points, stream_,
+               alloc_);
+    check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+
+  // Stage 1 of type 1: zero the fine grid, then spread.
+  void spread(const Cplx<T>* in) {
+    if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
+    launch_fill_zero<T>(grid_.data(), grid_.size(), stream_);
+    launch_spread<T, DIM>(ks_.ns, pts_.recs(), pts_.perm(), pts_.subs(), pts_.nsub(), in, nullptr,
+                          grid_.data(), geom_, ks_.es, stream_);
+  }
+
+  // Stage 3 of type 2: interpolate the current fine grid.
+  void interp(Cplx<T>* out) {
+    if (pts_.size() && !out) throw neonufft::InputError("Neonufft:: null output");
+    launch_interp<T, DIM>(ks_.ns, pts_.recs(), pts_.perm(), pts_.subs(), pts_.nsub(), grid_.data(),
+                          nullptr, out, geom_, ks_.es, stream_);
+  }
+
+  void transform_type_1(const Cplx<T>* in, Cplx<T>* out, const IntType* out_strides) {
+    if (!out || !out_strides) throw neonufft::InputError("Neonufft:: null output");
+    spread(in);
+    fft_.execute(grid_.data());
+    T* fs[3] = {fs_[0].data(), fs_[1].data(), fs_[2].data()};
+    launch_downsample<T, DIM>(order_fft(), grid_.data(), n_, fs, out, out_strides, modes_, stream_);
+  }
+
+  void transform_type_2(const Cplx<T>* in, const IntType* in_strides, Cplx<T>* out) {
+    if (!in || !in_strides) throw neonufft::InputError("Neonufft:: null input");
+    T* fs[3] = {fs_[0].data(), fs_[1].data(), fs_[2].data()};
+    launch_upsample<T, DIM>(order_fft(), in, in_strides, modes_, fs, grid_.data(), n_, stream_);
+    fft_.execute(grid_.data());
+    interp(out);
+  }
+
+  void info(nnf_plan_info* info) const {
+    *info = nnf_plan_info{};
+    info->precision = std::is_same_v<T, float> ? NNF_F32 : NNF_F64;
+    info->dim = DIM;
+    info->n_spread = ks_.ns;
+    info->es_beta = ks_.beta;
+    for (int d = 0; d < 3; ++d) {
+      info->modes[d] = modes_[d];
+      info->grid[d] = n_[d];
+      info->spread_grid[d] = n_[d];
+      info->bin_size[d] = geom_.bin[d];
+    }
+    info->num_bins = static_cast<int64_t>(pts_.nbins());
+    info->num_subproblems = static_cast<int64_t>(pts_.nsub());
+    info->num_points = static_cast<int64_t>(pts_.size());
+    info->horner_degree = ks_.horner_deg;
+    info->quad_nodes = ks_.quad.nq;
+  }
+
+  Cplx<T>* fine_grid() const { return grid_.data(); }
+  const T* fseries(int d) const { return d >= 0 && d < DIM ? fs_[d].data() : nullptr; }
+
+private:
+  int order_fft() const { return opt_.order == NEONUFFT_MODE_ORDER_FFT ? 1 : 0; }
+
+  neonufft::Options opt_;
+  int sign_;
+  hipStream_t stream_;
+  std::shared_ptr<neonufft::Allocator> alloc_;
+  PlanTuning tune_;
+  KernelSetup ks_;
+  IntType modes_[3];
+  IntType n_[3];
+  BinGeom geom_{};
+  DeviceBuffer<Cplx<T>> grid_;
+  FftPlan fft_;
+  DeviceBuffer<T> fs_[3];
+  PointSet<T, DIM> pts_;
+};
+
+template <typename T, int DIM>
+class PlanT3Impl {
+public:
+  // sizeof(complex T) * (spread grid + 2 * FFT grid), the reference's formula
+  // (gpu/plan_t3_impl.hpp:54-72).
+  static uint64_t grid_memory_size(const neonufft::Options& opt, const double* in_min,
+                                   const double* in_max, const double* out_min,
+                                   const double* out_max) {
+    int ns;
+    double beta;
+    kernel_width_and_beta(opt.tol, opt.upsampfac, &ns, &beta);
+    const GridInfoT3 gi = make_grid_info_t3(DIM, ns, opt.upsampfac, opt.recenter_threshold, in_min,
+                                            in_max, out_min, out_max);
+    uint64_t sg = 1, fg = 1;
+    for (int d = 0; d < DIM; ++d) {
+      sg *= static_cast<uint64_t>(gi.spread_grid_size[d]);
+      fg *= static_cast<uint64_t>(gi.fft_grid_size[d]);
+    }
+    return sizeof(Cplx<T>) * (sg + 2 * fg);
+  }
+
+  PlanT3Impl(const neonufft::Options& opt, int sign, IntType num_in, const T* const* in_pts,
+             IntType num_out, const T* const* out_pts, hipStream_t stream, IntType batch,
+             std::shared_ptr<neonufft::Allocator> alloc, const PlanTuning& tune = PlanTuning())
+      : opt_(opt), sign_(sign), stream_(stream), batch_(batch), alloc_(std::move(alloc)), tune_(tune) {
+    check_common();
+    if (num_in < 0 || num_out < 0) throw neonufft::InputError("Neonufft:: negative point count");
+    double bounds[4][3];
+    minmax(num_in, in_pts, bounds[0], bounds[1]);
+    minmax(num_out, out_pts, bounds[2], bounds[3]);
+    init(bounds[0], bounds[1], bounds[2], bounds[3]);
+    set_input_points(num_in, in_pts);
+    set_output_points(num_out, out_pts);
+  }
+
+  PlanT3Impl(const neonufft::Options& opt, int sign, const double* in_min, const double* in_max,
+             const double* out_min, const double* out_max, hipStream_t stream, IntType batch,
+             std::shared_ptr<neonufft::Allocator> alloc, const PlanTuning& tune = PlanTuning())
+      : opt_(opt), sign_(sign), stream_(stream), batch_(batch), alloc_(std::move(alloc)), tune_(tune) {
+    check_common();
+    init(in_min, in_max, out_min, out_max);
+  }
+
+  void set_input_points(IntType num_in, const T* const* pts) {
+    if (num_in < 0) throw neonufft::InputError("Neonufft:: negative point count");
+    Loc<T> l = make_loc(pts);
+    FoldParams fp{};
+    for (int d = 0; d < 3; ++d) {
+      fp.offset[d] = d < DIM ? gi_.input_offsets[d] : 0.0;
+      fp.scale[d] = d < DIM ? 1.0 / gi_.gamma[d] : 1.0;
+    }
+    in_pts_.build(l, static_cast<std::size_t>(num_in), fp, geom_in_, ks_.ns, tune_.max_points,
+                  stream_, alloc_);
+    has_pre_ = false;
+    for (int d = 0; d < DIM; ++d) has_pre_ = has_pre_ || gi_.output_offsets[d] != 0.0;
+    if (has_pre_) {
+      pre_.ensure(num_in ? static_cast<std::size_t>(num_in) : 1, alloc_);
+      launch_prephase<T, DIM>(sign_, l, in_pts_.perm(), static_cast<std::size_t>(num_in),
+                              gi_.output_offsets, pre_.data(), stream_);
+    }
+  }
+
+  void set_output_points(IntType num_out, const T* const* pts) {
+    if (num_out < 0) throw neonufft::InputError("Neonufft:: negative point count");
+    Loc<T> l = make_loc(pts);
+    FoldParams fp{};
+    double scale[3] = {1, 1, 1};
+    for (int d = 0; d < 3; ++d) {
+      fp.offset[d] = d < DIM ? gi_.output_offsets[d] : 0.0;
+      scale[d] = d < DIM ? gi_.gamma[d] * gi_.h[d] : 1.0;
+      fp.scale[d] = scale[d];
+    }
+    out_pts_.build(l, static_cast<std::size_t>(num_out), fp, geom_out_, ks_.ns, tune_.max_points,
+                   stream_, alloc_);
+    post_.ensure(num_out ? static_cast<std::size_t>(num_out) : 1, alloc_);
+    launch_postphase<T, DIM>(sign_, l, out_pts_.perm(), static_cast<std::size_t>(num_out),
+                             gi_.input_offsets, gi_.output_offsets, scale, ks_.quad, post_.data(),
+                             stream_);
+  }
+
+  void add_input(const Cplx<T>* in, IntType bdist) {
+    if (bdist <= 0) bdist = static_cast<IntType>(in_pts_.size());
+    if (in_pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
+    for (IntType b = 0; b < batch_; ++b) {
+      launch_spread<T, DIM>(ks_.ns, in_pts_.recs(), in_pts_.perm(), in_pts_.subs(), in_pts_.nsub(),
+                            in + b * bdist, has_pre_ ? pre_.data() : nullptr,
+                            spread_grid_.data() + b * spread_cells_, geom_in_, ks_.es, stream_);
+    }
+  }
+
+  void transform(Cplx<T>* out, IntType bdist) {
+    if (bdist <= 0) bdist = static_cast<IntType>(out_pts_.size());
+    if (out_pts_.size() && !out) throw neonufft::InputError("Neonufft:: null output");
+    T* fs[3] = {fs_[0].data(), fs_[1].data(), fs_[2].data()};
+    IntType strides[3] = {1, 1, 1};
+    for (int d = 1; d < DIM; ++d) strides[d] = strides[d - 1] * gi_.spread_grid_size[d - 1];
+    for (IntType b = 0; b < batch_; ++b) {
+      // The spread grid is read as CMCL-ordered modes (gpu/plan_t3_impl.hpp:227).
+      launch_upsample<T, DIM>(0, spread_grid_.data() + b * spread_cells_, strides,
+                              gi_.spread_grid_size, fs, fft_grid_.data(), gi_.fft_grid_size,
+                              stream_);
+      fft_.execute(fft_grid_.data());
+      launch_interp<T, DIM>(ks_.ns, out_pts_.recs(), out_pts_.perm(), out_pts_.subs(),
+                            out_pts_.nsub(), fft_grid_.data(), post_.data(), out + b * bdist,
+                            geom_out_, ks_.es, stream_);
+    }
+  }
+
+  void reset() {
+    launch_fill_zero<T>(spread_grid_.data(), spread_grid_.size(), stream_);
+  }
+
+  void info(nnf_plan_info* info) const {
+    *info = nnf_plan_info{};
+    info->precision = std::is_same_v<T, float> ? NNF_F32 : NNF_F64;
+    info->dim = DIM;
+    info->n_spread = ks_.ns;
+    info->es_beta = ks_.beta;
+    for (int d = 0; d < 3; ++d) {
+      info->modes[d] = gi_.spread_grid_size[d];
+      info->grid[d] = gi_.fft_grid_size[d];
+      info->spread_grid[d] = gi_.spread_grid_size[d];
+      info->bin_size[d] = geom_in_.bin[d];
+    }
+    info->num_bins = static_cast<int64_t>(in_pts_.nbins());
+    info->num_subproblems = static_cast<int64_t>(in_pts_.nsub());
+    info->num_points = static_cast<int64_t>(in_pts_.size());
+    info->num_out_points = static_cast<int64_t>(out_pts_.size());
+    info->horner_degree = ks_.horner_deg;
+    info->quad_nodes = ks_.quad.nq;
+  }
+
+  const GridInfoT3& grid_info() const { return gi_; }
+
+private:
+  void check_common() {
+    if (!alloc_) throw neonufft::InputError("Neonufft:: null allocator");
+    if (batch_ < 1) throw neonufft::InputError("Neonufft:: batch size must be >= 1");
+    ks_ = make_kernel_setup(opt_.tol, opt_.upsampfac, std::is_same_v<T, float>);
+  }
+
+  static Loc<T> make_loc(const T* const* pts) {
+    Loc<T> l{{nullptr, nullptr, nullptr}};
+    for (int d = 0; d < DIM; ++d) l.p[d] = pts ? pts[d] : nullptr;
+    return l;
+  }
+
+  // Per-dimension min/max of device coordinates, finished on the host
+  // (the reference does the same with hipCUB + a blocking copy,
+  // gpu/plan_t3_impl.hpp:88-102).
+  void minmax(IntType m, const T* const* pts, double* lo, double* hi) {
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = 0.0;
+      hi[d] = 0.0;
+    }
+    if (m <= 0) return;
+    const int nblk = minmax_blocks(static_cast<std::size_t>(m));
+    DeviceBuffer<double> part(static_cast<std::size_t>(2 * nblk * DIM), alloc_);
+    for (int d = 0; d < DIM; ++d) {
+      if (!pts || !pts[d]) throw neonufft::InputError("Neonufft:: null coordinate pointer");
+      launch_minmax<T>(pts[d], static_cast<std::size_t>(m), part.data() + 2 * nblk * d, stream_);
+    }
+    std::vector<double> host(static_cast<std::size_t>(2 * nblk * DIM));
+    check_hip(hipMemcpyAsync(host.data(), part.data(), host.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, stream_),
+              "hipMemcpyAsync(min/max)");
+    check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    for (int d = 0; d < DIM; ++d) {
+      double a = host[2 * nblk * d], b = host[2 * nblk * d + 1];
+      for (int k = 1; k < nblk; ++k) {
+        a = std::min(a, host[2 * nblk * d + 2 * k]);
+        b = std::max(b, host[2 * nblk * d + 2 * k + 1]);
+      }
+      lo[d] = a;
+      hi[d] = b;
+    }
+  }
+
+  void init(const double* in_min, const double* in_max, const double* out_min,
+            const double* out_max) {
+    gi_ = make_grid_info_t3(DIM, ks_.ns, opt_.upsampfac, opt_.recenter_threshold, in_min, in_max,
+                            out_min, out_max);
+    spread_cells_ = 1;
+    std::size_t fft_cells = 1;
+    for (int d = 0; d < DIM; ++d) {
+      spread_cells_ *= gi_.spread_grid_size[d];
+      fft_cells *= static_cast<std::size_t>(gi_.fft_grid_size[d]);
+    }
+    spread_grid_.reset(static_cast<std::size_t>(spread_cells_ * batch_), alloc_);
+    launch_fill_zero<T>(spread_grid_.data(), spread_grid_.size(), stream_);
+    fft_grid_.reset(fft_cells, alloc_);
+    fft_ = FftPlan(std::is_same_v<T, double>, DIM, gi_.fft_grid_size, sign_, 1, stream_, alloc_);
+    for (int d = 0; d < DIM; ++d) {
+      fs_[d].reset(static_cast<std::size_t>(gi_.fft_grid_size[d] / 2 + 1), alloc_);
+      launch_fseries<T>(fs_[d].data(), gi_.fft_grid_size[d], ks_.quad, stream_);
+    }
+    geom_in_ = choose_bins(DIM, gi_.spread_grid_size, ks_.ns, sizeof(Cplx<T>), tune_.bin);
+    geom_out_ = choose_bins(DIM, gi_.fft_grid_size, ks_.ns, sizeof(Cplx<T>), tune_.bin);
+    check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+
+  neonufft::Options opt_;
+  int sign_;
+  hipStream_t stream_;
+  IntType batch_;
+  std::shared_ptr<neonufft::Allocator> alloc_;
+  PlanTuning tune_;
+  KernelSetup ks_;
+  GridInfoT3 gi_;
+  IntType spread_cells_ = 1;
+  BinGeom geom_in_{}, geom_out_{};
+  DeviceBuffer<Cplx<T>> spread_grid_, fft_grid_, pre_, post_;
+  bool has_pre_ = false;
+  FftPlan fft_;
+  DeviceBuffer<T> fs_[3];
+  PointSet<T, DIM> in_pts_, out_pts_;
+};
+
+}  // namespace nnf

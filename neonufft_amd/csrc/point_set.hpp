@@ -1,0 +1,80 @@
+// A set of non-uniform points folded onto a grid and sorted by bin, with the
+// subproblem table the spreading / interpolation kernels consume.
+#pragma once
+
+#include <memory>
+
+#include "kernels.hpp"
+
+namespace nnf {
+
+template <typename T, int DIM>
+class PointSet {
+public:
+  // Folds, bins and sorts m points (loc[d][i]); synchronises `stream` once to
+  // size the subproblem table.
+  void build(Loc<T> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
+             uint32_t max_points, hipStream_t stream, const std::shared_ptr<neonufft::Allocator>& alloc) {
+    if (m > 0xffffffffull) {
+      throw neonufft::NotImplementedError("Neonufft:: more than 2^32 - 1 points per plan");
+    }
+    if (m > 0) {
+      for (int d = 0; d < DIM; ++d)
+        if (!loc.p[d]) throw neonufft::InputError("Neonufft:: null coordinate pointer");
+    }
+    g_ = g;
+    m_ = m;
+    max_points_ = max_points;
+    const std::size_t nbins = static_cast<std::size_t>(g.nbins[0]) * g.nbins[1] * g.nbins[2];
+    nbins_ = nbins;
+    count_.ensure(nbins + 1, alloc);
+    start_.ensure(nbins + 1, alloc);
+    nsubb_.ensure(nbins + 1, alloc);
+    sub_start_.ensure(nbins + 1, alloc);
+    scratch_.ensure(scan_scratch_size(nbins + 1), alloc);
+    recs_.ensure(m ? m : 1, alloc);
+    perm_.ensure(m ? m : 1, alloc);
+    rank_.ensure(m ? m : 1, alloc);
+
+    check_hip(hipMemsetAsync(count_.data(), 0, (nbins + 1) * sizeof(uint32_t), stream),
+              "hipMemsetAsync(bin counts)");
+    check_hip(hipMemsetAsync(nsubb_.data(), 0, (nbins + 1) * sizeof(uint32_t), stream),
+              "hipMemsetAsync(subproblem counts)");
+    launch_bin_count<T, DIM>(loc, m, fp, g, ns, count_.data(), rank_.data(), stream);
+    launch_exclusive_scan(count_.data(), start_.data(), nbins + 1, scratch_.data(), stream);
+    launch_subprob_counts(count_.data(), nbins, max_points, nsubb_.data(), stream);
+    launch_exclusive_scan(nsubb_.data(), sub_start_.data(), nbins + 1, scratch_.data(), stream);
+    uint32_t total = 0;
+    check_hip(hipMemcpyAsync(&total, sub_start_.data() + nbins, sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(subproblem total)");
+    check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    nsub_ = total;
+    subs_.ensure(nsub_ ? nsub_ : 1, alloc);
+    launch_subprob_build(count_.data(), start_.data(), sub_start_.data(), nbins, max_points,
+                         subs_.data(), stream);
+    launch_bin_scatter<T, DIM>(loc, m, fp, g, ns, start_.data(), rank_.data(), recs_.data(),
+                               perm_.data(), stream);
+  }
+
+  const PointRec<T, DIM>* recs() const { return recs_.data(); }
+  const uint32_t* perm() const { return perm_.data(); }
+  const Subprob* subs() const { return subs_.data(); }
+  const uint32_t* bin_counts() const { return count_.data(); }
+  const uint32_t* bin_starts() const { return start_.data(); }
+  std::size_t nsub() const { return nsub_; }
+  std::size_t size() const { return m_; }
+  std::size_t nbins() const { return nbins_; }
+  uint32_t max_points() const { return max_points_; }
+  const BinGeom& geom() const { return g_; }
+
+private:
+  BinGeom g_{};
+  std::size_t m_ = 0, nsub_ = 0, nbins_ = 0;
+  uint32_t max_points_ = 0;
+  DeviceBuffer<PointRec<T, DIM>> recs_;
+  DeviceBuffer<uint32_t> perm_, rank_, count_, start_, nsubb_, sub_start_, scratch_;
+  DeviceBuffer<Subprob> subs_;
+};
+
+}  // namespace nnf
