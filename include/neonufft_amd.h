@@ -150,8 +150,9 @@ typedef struct nnf_plan_info {
 int nnf_plan_get_info(nnf_plan plan, nnf_plan_info* info);
 int nnf_plan_t3_get_info(nnf_plan_t3 plan, nnf_plan_info* info);
 
-/* Zero the fine grid and spread `in` onto it (type-1 stage 1). */
-int nnf_plan_spread(nnf_plan plan, const void* in);
+/* Spread `in` onto the fine grid (type-1 stage 1). The grid is zeroed first
+ * unless `accumulate` is non-zero. */
+int nnf_plan_spread(nnf_plan plan, const void* in, int accumulate);
 /* Interpolate the current fine grid at the plan's points into `out`
  * (type-2 stage 3). */
 int nnf_plan_interp(nnf_plan plan, void* out);

@@ -193,8 +193,8 @@ class Plan:
                                                  ctypes.c_void_p(_ptr(out))))
 
     # ---- stage-level extension (no reference counterpart) ----
-    def spread(self, in_: Pointer) -> None:
-        _check(self._L.nnf_plan_spread(self._h, ctypes.c_void_p(_ptr(in_))))
+    def spread(self, in_: Pointer, accumulate: bool = False) -> None:
+        _check(self._L.nnf_plan_spread(self._h, ctypes.c_void_p(_ptr(in_)), int(accumulate)))
 
     def interp(self, out: Pointer) -> None:
         _check(self._L.nnf_plan_interp(self._h, ctypes.c_void_p(_ptr(out))))

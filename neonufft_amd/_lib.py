@@ -96,7 +96,7 @@ def lib():
     L.nnf_plan_t3_destroy.argtypes = [vp]
     L.nnf_plan_get_info.argtypes = [vp, ctypes.POINTER(NnfPlanInfo)]
     L.nnf_plan_t3_get_info.argtypes = [vp, ctypes.POINTER(NnfPlanInfo)]
-    L.nnf_plan_spread.argtypes = [vp, vp]
+    L.nnf_plan_spread.argtypes = [vp, vp, c_int]
     L.nnf_plan_interp.argtypes = [vp, vp]
     L.nnf_plan_fine_grid.argtypes = [vp, vpp]
     L.nnf_plan_fseries.argtypes = [vp, c_int, vpp]

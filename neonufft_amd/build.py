@@ -20,6 +20,8 @@ LIB = os.path.join(HERE, "libneonufft_amd.so")
 
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# host-only translation units (no kernels) go through the host compiler
+HOSTCXX = os.environ.get("HOSTCXX", "/opt/rocm/llvm/bin/clang++")
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
@@ -44,12 +46,13 @@ SOURCES = [
 ]
 
 
-def _compile(src: str, device: bool) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + COMMON + (DEVICE if device else ["-D__HIP_PLATFORM_AMD__"]) + [
-        "-c", os.path.join(CSRC, src), "-o", obj]
-    if not device:
-        cmd[1:1] = ["-x", "c++"] if False else []
+def _compile(src: str, device: bool, csrc: str, build_dir: str, extra) -> str:
+    obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+    if device:
+        cmd = [HIPCC] + COMMON + list(extra) + DEVICE
+    else:
+        cmd = [HOSTCXX] + COMMON + list(extra) + ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
+    cmd += ["-c", os.path.join(csrc, src), "-o", obj]
     t0 = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -57,29 +60,33 @@ def _compile(src: str, device: bool) -> str:
     return f"{src}: {time.time() - t0:.1f}s"
 
 
-def build(verbose: bool = True, jobs: int | None = None) -> str:
-    """Compiles every source and links libneonufft_amd.so. Always rebuilds."""
+def build(verbose: bool = True, jobs: int | None = None, csrc: str = CSRC,
+          build_dir: str = BUILD, lib: str = LIB, extra_flags=()) -> str:
+    """Compiles every source and links the shared library. Always rebuilds.
+
+    csrc / build_dir / lib / extra_flags let a developer build a modified copy
+    of the sources elsewhere; the defaults build the package in-tree."""
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(build_dir, exist_ok=True)
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(_compile, s, d) for s, d in SOURCES]
+        futs = [ex.submit(_compile, s, d, csrc, build_dir, extra_flags) for s, d in SOURCES]
         for f in futs:
             msg = f.result()
             if verbose:
                 print("  built", msg, flush=True)
-    objs = [os.path.join(BUILD, os.path.splitext(s)[0] + ".o") for s, _ in SOURCES]
-    tmp = LIB + ".tmp"
+    objs = [os.path.join(build_dir, os.path.splitext(s)[0] + ".o") for s, _ in SOURCES]
+    tmp = lib + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + [
         "-L/opt/rocm/lib", "-lrocfft", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("  linked", LIB, flush=True)
-    return LIB
+        print("  linked", lib, flush=True)
+    return lib
 
 
 if __name__ == "__main__":

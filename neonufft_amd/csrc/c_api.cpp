@@ -7,7 +7,7 @@
 #include <memory>
 #include <new>
 
-#include "../../include/neonufft_amd.h"
+#include <neonufft_amd.h>
 #include "plan_impl.hpp"
 
 namespace {
@@ -22,7 +22,7 @@ struct PlanIface {
   virtual void set_points(int64_t n, const void* const* loc) = 0;
   virtual void t1(const void* in, void* out, const int64_t* st) = 0;
   virtual void t2(const void* in, const int64_t* st, void* out) = 0;
-  virtual void spread(const void* in) = 0;
+  virtual void spread(const void* in, bool accumulate) = 0;
   virtual void interp(void* out) = 0;
   virtual void* grid() = 0;
   virtual const void* fseries(int d) = 0;
@@ -44,7 +44,9 @@ struct PlanHolder final : PlanIface {
   void t2(const void* in, const int64_t* st, void* out) override {
     impl.transform_type_2(static_cast<const Cplx<T>*>(in), st, static_cast<Cplx<T>*>(out));
   }
-  void spread(const void* in) override { impl.spread(static_cast<const Cplx<T>*>(in)); }
+  void spread(const void* in, bool accumulate) override {
+    impl.spread(static_cast<const Cplx<T>*>(in), accumulate);
+  }
   void interp(void* out) override { impl.interp(static_cast<Cplx<T>*>(out)); }
   void* grid() override { return impl.fine_grid(); }
   const void* fseries(int d) override { return impl.fseries(d); }
@@ -340,10 +342,10 @@ NEONUFFT_EXPORT int nnf_plan_t3_get_info(nnf_plan_t3 plan, nnf_plan_info* info) 
   });
 }
 
-NEONUFFT_EXPORT int nnf_plan_spread(nnf_plan plan, const void* in) {
+NEONUFFT_EXPORT int nnf_plan_spread(nnf_plan plan, const void* in, int accumulate) {
   return guarded([&] {
     NNF_REQUIRE_PLAN(plan);
-    plan->p->spread(in);
+    plan->p->spread(in, accumulate != 0);
   });
 }
 

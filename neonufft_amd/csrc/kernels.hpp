@@ -50,6 +50,9 @@ template <typename T>
 void launch_minmax(const T* x, std::size_t m, double* partial, hipStream_t stream);
 
 // ---- spread_interp.hip: ES spreading / interpolation on binned points -----
+// Upper bound on the points of one subproblem (one workgroup's worth).
+constexpr uint32_t kMaxSubproblemPoints = 1024;
+
 template <typename T, int DIM>
 void launch_spread(int ns, const PointRec<T, DIM>* pts, const uint32_t* perm, const Subprob* subs,
                    std::size_t nsub, const Cplx<T>* in, const Cplx<T>* pre_sorted, Cplx<T>* grid,

@@ -147,7 +147,7 @@ BinGeom choose_bins(int dim, const IntType* n, int ns, std::size_t elem_bytes,
     for (int d = 0; d < dim; ++d)
       if (bin_override[d] > 0) pref[d] = bin_override[d];
   }
-  const std::size_t budget = 64 * 1024;
+  const std::size_t budget = 60 * 1024;  // LDS tile bytes per workgroup
   auto bytes = [&](const int* b) {
     std::size_t cells = 1;
     for (int d = 0; d < dim; ++d) cells *= static_cast<std::size_t>(b[d] + ns);

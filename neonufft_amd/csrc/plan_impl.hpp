@@ -15,7 +15,7 @@
 #include <memory>
 #include <vector>
 
-#include "../../include/neonufft_amd.h"
+#include <neonufft_amd.h>
 #include "fft.hpp"
 #include "params.hpp"
 #include "point_set.hpp"
@@ -69,10 +69,10 @@ public:
     check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   }
 
-  // Stage 1 of type 1: zero the fine grid, then spread.
-  void spread(const Cplx<T>* in) {
+  // Stage 1 of type 1: zero the fine grid (unless accumulating), then spread.
+  void spread(const Cplx<T>* in, bool accumulate = false) {
     if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
-    launch_fill_zero<T>(grid_.data(), grid_.size(), stream_);
+    if (!accumulate) launch_fill_zero<T>(grid_.data(), grid_.size(), stream_);
     launch_spread<T, DIM>(ks_.ns, pts_.recs(), pts_.perm(), pts_.subs(), pts_.nsub(), in, nullptr,
                           grid_.data(), geom_, ks_.es, stream_);
   }

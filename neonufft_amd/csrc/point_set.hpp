@@ -24,6 +24,7 @@ public:
     }
     g_ = g;
     m_ = m;
+    if (max_points == 0 || max_points > kMaxSubproblemPoints) max_points = kMaxSubproblemPoints;
     max_points_ = max_points;
     const std::size_t nbins = static_cast<std::size_t>(g.nbins[0]) * g.nbins[1] * g.nbins[2];
     nbins_ = nbins;

@@ -188,14 +188,17 @@ __global__ void __launch_bounds__(kBlock)
                          uint32_t max_points, Subprob* __restrict__ subs) {
   for (std::size_t b = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock; b < nbins;
        b += static_cast<std::size_t>(gridDim.x) * kBlock) {
+    // split the bin into nsub near-equal chunks of at most max_points
     const uint32_t c = count[b];
     const uint32_t s0 = sub_start[b];
     const uint32_t b0 = bin_start[b];
-    for (uint32_t k = 0; k * max_points < c; ++k) {
+    const uint32_t ns = (c + max_points - 1) / max_points;
+    const uint32_t chunk = ns ? (c + ns - 1) / ns : 0;
+    for (uint32_t k = 0; k < ns; ++k) {
       Subprob sp;
       sp.bin = static_cast<uint32_t>(b);
-      sp.begin = b0 + k * max_points;
-      sp.count = min(max_points, c - k * max_points);
+      sp.begin = b0 + k * chunk;
+      sp.count = min(chunk, c - k * chunk);
       sp.pad = 0;
       subs[s0 + k] = sp;
     }
