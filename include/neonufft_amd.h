@@ -125,6 +125,20 @@ int nnf_plan_t3_reset(nnf_plan_t3 plan);
 
 int nnf_plan_t3_destroy(nnf_plan_t3 plan);
 
+/* ---- Tuning extension (no reference counterpart) -------------------------
+ * The subgrid tile (bin) edge per dimension and the maximum number of points
+ * one workgroup handles. Zero entries keep the built-in defaults. */
+typedef struct nnf_tuning {
+  int32_t bin_size[3];
+  int32_t max_points;
+  int32_t reserved[4];
+} nnf_tuning;
+
+/* nnf_plan_create with explicit tuning (NULL = defaults). */
+int nnf_plan_create_tuned(nnf_plan* plan, int precision, int dim, const nnf_options* opt,
+                          int sign, int64_t num_nu, const void* const* loc, const int64_t* modes,
+                          void* stream, const nnf_tuning* tuning);
+
 /* ---- Stage-level extension (no reference counterpart) --------------------
  * The reference runs these stages only inside transform_type_{1,2}
  * (src/neonufft/gpu/plan_impl.hpp:91-136). They are exposed for stage-level

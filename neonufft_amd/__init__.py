@@ -159,7 +159,9 @@ class Plan:
     """
 
     def __init__(self, opt: Options, sign: int, num_nu: int, loc: Sequence[Pointer],
-                 modes: Sequence[int], stream=None, dtype=None):
+                 modes: Sequence[int], stream=None, dtype=None, bin_size=None,
+                 max_points: int = 0):
+        """bin_size / max_points: optional tuning (extension; None/0 = defaults)."""
         self._L = _L.lib()
         self.dim = len(modes)
         self.modes = [int(m) for m in modes]
@@ -167,9 +169,17 @@ class Plan:
         self.num_nu = int(num_nu)
         self._h = ctypes.c_void_p()
         vpp, _keep = _vpp([_ptr(p) for p in loc])
-        _check(self._L.nnf_plan_create(ctypes.byref(self._h), self.precision, self.dim,
-                                       ctypes.byref(opt._c()), int(sign), self.num_nu, vpp,
-                                       _i64p(self.modes), ctypes.c_void_p(_stream(stream))))
+        tune = None
+        if bin_size is not None or max_points:
+            tune = _L.NnfTuning()
+            for d, b in enumerate(list(bin_size or [])[:3]):
+                tune.bin_size[d] = int(b)
+            tune.max_points = int(max_points)
+            tune = ctypes.byref(tune)
+        _check(self._L.nnf_plan_create_tuned(ctypes.byref(self._h), self.precision, self.dim,
+                                             ctypes.byref(opt._c()), int(sign), self.num_nu,
+                                             vpp, _i64p(self.modes),
+                                             ctypes.c_void_p(_stream(stream)), tune))
 
     def set_points(self, num_nu: int, loc: Sequence[Pointer]) -> None:
         vpp, _keep = _vpp([_ptr(p) for p in loc])

@@ -16,7 +16,7 @@ EXPORTED = (
     "nnf_plan_t3_set_input_points", "nnf_plan_t3_set_output_points", "nnf_plan_t3_add_input",
     "nnf_plan_t3_transform", "nnf_plan_t3_reset", "nnf_plan_t3_destroy",
     "nnf_plan_get_info", "nnf_plan_t3_get_info", "nnf_plan_spread", "nnf_plan_interp",
-    "nnf_plan_fine_grid", "nnf_plan_fseries",
+    "nnf_plan_fine_grid", "nnf_plan_fseries", "nnf_plan_create_tuned",
 )
 
 
@@ -31,6 +31,14 @@ class NnfOptions(ctypes.Structure):
         ("kernel_approximation", ctypes.c_int),
         ("order", ctypes.c_int),
         ("kernel_type", ctypes.c_int),
+    ]
+
+
+class NnfTuning(ctypes.Structure):
+    _fields_ = [
+        ("bin_size", ctypes.c_int32 * 3),
+        ("max_points", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -79,6 +87,8 @@ def lib():
     L.nnf_last_error.restype = ctypes.c_char_p
     L.nnf_version.restype = ctypes.c_char_p
     L.nnf_plan_create.argtypes = [vpp, c_int, c_int, optp, c_int, i64, vpp, i64p, vp]
+    L.nnf_plan_create_tuned.argtypes = [vpp, c_int, c_int, optp, c_int, i64, vpp, i64p, vp,
+                                        ctypes.POINTER(NnfTuning)]
     L.nnf_plan_set_points.argtypes = [vp, i64, vpp]
     L.nnf_plan_transform_type_1.argtypes = [vp, vp, vp, i64p]
     L.nnf_plan_transform_type_2.argtypes = [vp, vp, i64p, vp]

@@ -33,8 +33,9 @@ template <typename T, int DIM>
 struct PlanHolder final : PlanIface {
   nnf::PlanImpl<T, DIM> impl;
   PlanHolder(const neonufft::Options& o, int sign, IntType m, const void* const* loc,
-             const int64_t* modes, hipStream_t s, std::shared_ptr<neonufft::Allocator> a)
-      : impl(o, sign, m, reinterpret_cast<const T* const*>(loc), modes, s, std::move(a)) {}
+             const int64_t* modes, hipStream_t s, std::shared_ptr<neonufft::Allocator> a,
+             const nnf::PlanTuning& t)
+      : impl(o, sign, m, reinterpret_cast<const T* const*>(loc), modes, s, std::move(a), t) {}
   void set_points(int64_t n, const void* const* loc) override {
     impl.set_points(n, reinterpret_cast<const T* const*>(loc));
   }
@@ -185,19 +186,36 @@ NEONUFFT_EXPORT const char* nnf_last_error(void) { return g_last_error.c_str(); 
 
 NEONUFFT_EXPORT const char* nnf_version(void) { return NEONUFFT_VERSION; }
 
-NEONUFFT_EXPORT int nnf_plan_create(nnf_plan* plan, int precision, int dim, const nnf_options* opt,
-                                    int sign, int64_t num_nu, const void* const* loc,
-                                    const int64_t* modes, void* stream) {
+NEONUFFT_EXPORT int nnf_plan_create_tuned(nnf_plan* plan, int precision, int dim,
+                                          const nnf_options* opt, int sign, int64_t num_nu,
+                                          const void* const* loc, const int64_t* modes,
+                                          void* stream, const nnf_tuning* tuning) {
   return guarded([&] {
     if (!plan) throw neonufft::InputError("Neonufft:: null plan output pointer");
     *plan = nullptr;
     if (!modes) throw neonufft::InputError("Neonufft:: null modes");
+    nnf::PlanTuning t;
+    if (tuning) {
+      for (int d = 0; d < 3; ++d) {
+        if (tuning->bin_size[d] < 0) throw neonufft::InputError("Neonufft:: negative bin size");
+        t.bin[d] = tuning->bin_size[d];
+      }
+      if (tuning->max_points < 0) throw neonufft::InputError("Neonufft:: negative max_points");
+      if (tuning->max_points > 0) t.max_points = static_cast<uint32_t>(tuning->max_points);
+    }
     auto h = std::make_unique<nnf_plan_s>();
     h->p.reset(make_holder<PlanHolder, PlanIface>(
         precision, dim, to_options(opt), sign, static_cast<IntType>(num_nu), loc, modes,
-        static_cast<hipStream_t>(stream), default_alloc()));
+        static_cast<hipStream_t>(stream), default_alloc(), t));
     *plan = h.release();
   });
+}
+
+NEONUFFT_EXPORT int nnf_plan_create(nnf_plan* plan, int precision, int dim, const nnf_options* opt,
+                                    int sign, int64_t num_nu, const void* const* loc,
+                                    const int64_t* modes, void* stream) {
+  return nnf_plan_create_tuned(plan, precision, dim, opt, sign, num_nu, loc, modes, stream,
+                               nullptr);
 }
 
 NEONUFFT_EXPORT int nnf_plan_set_points(nnf_plan plan, int64_t num_nu, const void* const* loc) {

@@ -22,6 +22,9 @@ struct QuadRule {
   int nq;
 };
 
+// Upper bound on the points of one subproblem (one workgroup's worth).
+constexpr uint32_t kMaxSubproblemPoints = 2048;
+
 // ---- points.hip: fold / rescale, counting sort by bin, subproblems -------
 template <typename T, int DIM>
 void launch_bin_count(Loc<T> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
@@ -44,15 +47,18 @@ void launch_subprob_build(const uint32_t* count, const uint32_t* bin_start,
                           const uint32_t* sub_start, std::size_t nbins, uint32_t max_points,
                           Subprob* subs, hipStream_t stream);
 
+// Reorders the points inside every subproblem for conflict-free LDS access
+// (bank-balanced waves); needs the subproblem table.
+template <typename T, int DIM>
+void launch_bank_order(const Subprob* subs, std::size_t nsub, PointRec<T, DIM>* pts,
+                       uint32_t* perm, const BinGeom& g, hipStream_t stream);
+
 // Per-block partial min/max of x (as double): partial[2*b] = min, [2*b+1] = max.
 int minmax_blocks(std::size_t m);
 template <typename T>
 void launch_minmax(const T* x, std::size_t m, double* partial, hipStream_t stream);
 
 // ---- spread_interp.hip: ES spreading / interpolation on binned points -----
-// Upper bound on the points of one subproblem (one workgroup's worth).
-constexpr uint32_t kMaxSubproblemPoints = 1024;
-
 template <typename T, int DIM>
 void launch_spread(int ns, const PointRec<T, DIM>* pts, const uint32_t* perm, const Subprob* subs,
                    std::size_t nsub, const Cplx<T>* in, const Cplx<T>* pre_sorted, Cplx<T>* grid,

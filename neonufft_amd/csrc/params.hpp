@@ -44,7 +44,7 @@ BinGeom choose_bins(int dim, const IntType* n, int ns, std::size_t elem_bytes,
                     const int* bin_override = nullptr);
 
 // Default number of points per subproblem.
-constexpr uint32_t kDefaultMaxPoints = kMaxSubproblemPoints;
+constexpr uint32_t kDefaultMaxPoints = 2048;
 
 // Type-3 grid sizing: src/neonufft/util/grid_info_t3.hpp:18-52 and
 // src/contrib/es_kernel/util.hpp:39-105 of the reference, in FP64.

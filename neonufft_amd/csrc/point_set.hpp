@@ -56,6 +56,7 @@ public:
                          subs_.data(), stream);
     launch_bin_scatter<T, DIM>(loc, m, fp, g, ns, start_.data(), rank_.data(), recs_.data(),
                                perm_.data(), stream);
+    launch_bank_order<T, DIM>(subs_.data(), nsub_, recs_.data(), perm_.data(), g, stream);
   }
 
   const PointRec<T, DIM>* recs() const { return recs_.data(); }

@@ -205,6 +205,71 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Reorders the points of each subproblem so that every consecutive run of 32
+// points has distinct LDS tile base addresses modulo 32 (as far as the counts
+// allow): points are sorted by (rank within residue class, residue), where
+// residue = tile-linear index of the point's first kernel cell mod 32. The
+// spreading / interpolation kernels give consecutive points to consecutive
+// lanes and all lanes step through the same kernel-cell offsets, so a wave
+// whose bases are spread over the residues touches every LDS bank equally
+// at every step instead of piling onto a few banks.
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderPerThread = (kMaxSubproblemPoints + kOrderThreads - 1) / kOrderThreads;
+
+template <typename T, int DIM>
+__device__ inline uint32_t tile_residue(const PointRec<T, DIM>& rec, int t0, int t1) {
+  int base = 0;
+  if constexpr (CellPack<DIM>::bits == 32) {
+    base = static_cast<int>(rec.cell);
+  } else {
+    const int r0 = static_cast<int>(rec.cell & CellPack<DIM>::mask);
+    const int r1 = static_cast<int>((rec.cell >> CellPack<DIM>::bits) & CellPack<DIM>::mask);
+    int r2 = 0;
+    if constexpr (DIM == 3)
+      r2 = static_cast<int>((rec.cell >> (2 * CellPack<DIM>::bits)) & CellPack<DIM>::mask);
+    base = (r2 * t1 + r1) * t0 + r0;
+  }
+  return static_cast<uint32_t>(base) & 31u;
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kOrderThreads)
+    bank_order_kernel(const Subprob* __restrict__ subs, PointRec<T, DIM>* __restrict__ pts,
+                      uint32_t* __restrict__ perm, int t0, int t1) {
+  __shared__ uint32_t cnt[32];
+  if (threadIdx.x < 32) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const Subprob sp = subs[blockIdx.x];
+  PointRec<T, DIM> rec[kOrderPerThread];
+  uint32_t idx[kOrderPerThread], res[kOrderPerThread], rank[kOrderPerThread];
+#pragma unroll
+  for (int k = 0; k < kOrderPerThread; ++k) {
+    const uint32_t i = threadIdx.x + k * kOrderThreads;
+    if (i < sp.count) {
+      rec[k] = pts[sp.begin + i];
+      idx[k] = perm[sp.begin + i];
+      res[k] = tile_residue<T, DIM>(rec[k], t0, t1);
+      rank[k] = atomicAdd(&cnt[res[k]], 1u);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kOrderPerThread; ++k) {
+    const uint32_t i = threadIdx.x + k * kOrderThreads;
+    if (i < sp.count) {
+      uint32_t pos = 0;
+#pragma unroll
+      for (uint32_t r = 0; r < 32; ++r) {
+        const uint32_t c = cnt[r];
+        pos += min(c, rank[k]);
+        if (r < res[k] && c > rank[k]) ++pos;
+      }
+      pts[sp.begin + pos] = rec[k];
+      perm[sp.begin + pos] = idx[k];
+    }
+  }
+}
+
 constexpr int kMinMaxBlocks = 512;
 
 template <typename T>
@@ -253,6 +318,15 @@ void launch_bin_scatter(Loc<T> loc, std::size_t m, const FoldParams& fp, const B
   hipLaunchKernelGGL((bin_scatter_kernel<T, DIM>), dim3(grid_for(m, kBlock)), dim3(kBlock), 0,
                      stream, loc, m, fp, g, ns, bin_start, rank, pts, perm);
   check_launch("bin_scatter_kernel");
+}
+
+template <typename T, int DIM>
+void launch_bank_order(const Subprob* subs, std::size_t nsub, PointRec<T, DIM>* pts,
+                       uint32_t* perm, const BinGeom& g, hipStream_t stream) {
+  if (nsub == 0) return;
+  hipLaunchKernelGGL((bank_order_kernel<T, DIM>), dim3(static_cast<unsigned>(nsub)),
+                     dim3(kOrderThreads), 0, stream, subs, pts, perm, g.tile[0], g.tile[1]);
+  check_launch("bank_order_kernel");
 }
 
 std::size_t scan_scratch_size(std::size_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
@@ -306,7 +380,9 @@ void launch_minmax(const T* x, std::size_t m, double* partial, hipStream_t strea
   template void launch_bin_scatter<T, DIM>(Loc<T>, std::size_t, const FoldParams&,              \
                                            const BinGeom&, int, const uint32_t*,                \
                                            const uint32_t*, PointRec<T, DIM>*, uint32_t*,       \
-                                           hipStream_t);
+                                           hipStream_t);                                        \
+  template void launch_bank_order<T, DIM>(const Subprob*, std::size_t, PointRec<T, DIM>*,       \
+                                          uint32_t*, const BinGeom&, hipStream_t);
 
 NNF_INST_POINTS(float, 1)
 NNF_INST_POINTS(float, 2)

@@ -129,7 +129,13 @@ struct FixedAcc<double> {
   static constexpr int bits = 62;
 };
 
-__device__ inline int to_fixed(float v) { return __float2int_rn(v); }
+// Round to the nearest integer (floor(v + 0.5)) in one VALU instruction; the
+// compiler emits rndne + cvt (two) for __float2int_rn. |v| <= 2^30 here.
+__device__ inline int to_fixed(float v) {
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+}
 __device__ inline long long to_fixed(double v) { return __double2ll_rn(v); }
 
 __device__ inline void lds_add_fixed(int* p, int v) {
@@ -160,13 +166,16 @@ __global__ void __launch_bounds__(kThreads)
                   Cplx<T>* __restrict__ grid, BinGeom g, EsParams es) {
   using Acc = typename FixedAcc<T>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  Acc* tile = reinterpret_cast<Acc*>(smem_raw);
   __shared__ T wsum[kThreads / 64];
 
   const Subprob sp = subs[xcd_remap(blockIdx.x, nsub)];
   const TileInfo<DIM, NS> ti(sp.bin, g);
   const int tile_cells = g.tile[0] * (DIM >= 2 ? g.tile[1] : 1) * (DIM >= 3 ? g.tile[2] : 1);
-  for (int i = threadIdx.x; i < 2 * tile_cells; i += kThreads) tile[i] = Acc(0);
+  // Real and imaginary parts live in separate planes so that 32-bit atomics
+  // of neighbouring cells fall in neighbouring banks.
+  Acc* tre = reinterpret_cast<Acc*>(smem_raw);
+  Acc* tim = tre + tile_cells;
+  for (int i = threadIdx.x; i < 2 * tile_cells; i += kThreads) tre[i] = Acc(0);
 
   // This thread's points (at most kPointsPerThread) and the tile bound.
   const uint32_t end = sp.begin + sp.count;
@@ -203,8 +212,8 @@ __global__ void __launch_bounds__(kThreads)
     if constexpr (DIM == 1) {
 #pragma unroll
       for (int a = 0; a < NS; ++a) {
-        lds_add_fixed(&tile[2 * (r0 + a)], to_fixed(kx[a] * c.x));
-        lds_add_fixed(&tile[2 * (r0 + a) + 1], to_fixed(kx[a] * c.y));
+        lds_add_fixed(&tre[r0 + a], to_fixed(kx[a] * c.x));
+        lds_add_fixed(&tim[r0 + a], to_fixed(kx[a] * c.y));
       }
     } else {
       T ky[NS];
@@ -225,8 +234,8 @@ __global__ void __launch_bounds__(kThreads)
           const int base = ((r2 + cz) * ti.t1 + r1 + b) * ti.t0 + r0;
 #pragma unroll
           for (int a = 0; a < NS; ++a) {
-            lds_add_fixed(&tile[2 * (base + a)], to_fixed(kx[a] * vr));
-            lds_add_fixed(&tile[2 * (base + a) + 1], to_fixed(kx[a] * vi));
+            lds_add_fixed(&tre[base + a], to_fixed(kx[a] * vr));
+            lds_add_fixed(&tim[base + a], to_fixed(kx[a] * vi));
           }
         }
       }
@@ -249,8 +258,8 @@ __global__ void __launch_bounds__(kThreads)
         (static_cast<std::size_t>(gz) * static_cast<std::size_t>(g.n[1]) + gy) * g.n[0];
     const int trow = (i2 * ti.t1 + i1) * ti.t0;
     for (int i0 = lane; i0 < ti.ext[0]; i0 += 64) {
-      const Acc qr = tile[2 * (trow + i0)];
-      const Acc qi = tile[2 * (trow + i0) + 1];
+      const Acc qr = tre[trow + i0];
+      const Acc qi = tim[trow + i0];
       if (qr != Acc(0) || qi != Acc(0)) {
         T* dst = reinterpret_cast<T*>(grid + grow + wrap_once(ti.org[0] + i0, g.n[0]));
         hbm_add(dst, static_cast<T>(qr) * inv);
