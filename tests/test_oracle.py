@@ -1,0 +1,317 @@
+"""Pins the FP64 oracle (oracle/) before it is trusted as the checker.
+
+Every oracle function is checked against something other than its own
+formula: the reference-documented parameter table, closed forms, numpy /
+scipy routines, brute-force sums, the committed golden fixtures, and
+end-to-end agreement of the NUFFT pipeline with the direct NUDFT. The
+planted-mistake tests at the end show that these checks fail for the
+plausible errors (wrong sign, dropped (-1)^k, transposed axes).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# ---------------------------------------------------------------------------
+# Kernel parameters: src/neonufft/es_kernel_param.hpp:24-56
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("tol,ns", [(1e-1, 2), (1e-2, 3), (1e-3, 4), (1e-4, 5), (1e-5, 6),
+                                    (1e-6, 7), (1e-7, 8), (1e-8, 9), (1e-9, 10), (1e-10, 11),
+                                    (1e-11, 12), (1e-12, 13), (1e-13, 14), (1e-14, 15),
+                                    (1e-16, 16), (1.0, 2)])
+def test_kernel_width_sigma2(tol, ns):
+    # sigma = 2: w = digits + 1 (the fiNUFFT rule the reference keeps), clamped to [2, 16]
+    got_ns, beta = O.kernel_params(tol, 2.0)
+    assert got_ns == ns
+    fac = {2: 2.20, 3: 2.26, 4: 2.38}.get(ns, 2.30)
+    assert beta == pytest.approx(fac * ns, rel=1e-15)
+
+
+def test_kernel_table_of_the_configs():
+    # tol -> (ns, beta) for the benchmark configurations
+    assert O.kernel_params(1e-9, 2.0)[0] == 10 and O.kernel_params(1e-9, 2.0)[1] == pytest.approx(23.0)
+    assert O.kernel_params(1e-5, 2.0) == (6, pytest.approx(13.8))
+    assert O.kernel_params(1e-4, 2.0) == (5, pytest.approx(11.5))
+    assert O.kernel_params(1e-6, 2.0) == (7, pytest.approx(16.1))
+
+
+def test_kernel_width_low_upsampling():
+    # sigma != 2: ns = ceil(-ln tol / (pi sqrt(1 - 1/sigma))), beta = 0.97 pi (1 - 1/(2 sigma)) ns
+    for sigma, tol in [(1.25, 1e-5), (1.6, 1e-5), (1.25, 1e-3), (1.5, 1e-9)]:
+        ns, beta = O.kernel_params(tol, sigma)
+        exact = -math.log(tol) / (math.pi * math.sqrt(1 - 1 / sigma))
+        assert ns - 1 < exact <= ns or ns in (2, 16)
+        assert beta / ns == pytest.approx(0.97 * math.pi * (1 - 0.5 / sigma), rel=1e-14)
+    assert O.kernel_params(1e-5, 1.25)[0] == 9
+
+
+# ---------------------------------------------------------------------------
+# ES kernel: closed-form values
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("ns,beta", [(6, 13.8), (7, 16.1), (9, 16.46), (4, 9.52)])
+def test_es_kernel_closed_form(ns, beta):
+    assert O.es_kernel([0.0], ns, beta)[0] == 1.0
+    # (2t/ns)^2 = 3/4 -> sqrt(1 - 3/4) = 1/2 -> exp(-beta/2)
+    t = ns / 2 * math.sqrt(0.75)
+    assert O.es_kernel([t], ns, beta)[0] == pytest.approx(math.exp(-beta / 2), rel=1e-13)
+    # support edge and outside
+    assert O.es_kernel([ns / 2, -ns / 2, ns / 2 + 0.3], ns, beta).tolist() == [0.0, 0.0, 0.0]
+    ts = np.linspace(0, ns / 2 - 1e-9, 200)
+    v = O.es_kernel(ts, ns, beta)
+    assert np.all(np.diff(v) < 0)  # strictly decreasing on [0, ns/2)
+    assert np.allclose(O.es_kernel(-ts, ns, beta), v, rtol=0, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# Gauss-Legendre and the Fourier transform of the kernel
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 12, 24, 40, 64])
+def test_gauss_legendre_matches_numpy(n):
+    x, w = O.gauss_legendre(n)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(x, xr, rtol=0, atol=1e-14)
+    assert np.allclose(w, wr, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("ns,beta", [(6, 13.8), (7, 16.1), (9, 16.46), (16, 36.8)])
+def test_phihat_matches_adaptive_quadrature(ns, beta):
+    from scipy import integrate
+    hw = ns / 2
+    phi = lambda t: math.exp(beta * (math.sqrt(max(0.0, 1 - (t / hw) ** 2)) - 1))
+    for f in [0.0, 0.3, 1.1, math.pi / 2, 2.5]:
+        ref, _ = integrate.quad(lambda t: phi(t) * math.cos(f * t), -hw, hw, limit=400,
+                                epsabs=1e-15, epsrel=1e-13)
+        got = O.phihat(f, ns, beta)
+        # the sqrt branch point at +-ns/2 (where phi ~ e^-beta) limits Gauss-Legendre
+        assert abs(got - ref) <= 1e-6 * math.exp(-beta / 4) * abs(O.phihat(0.0, ns, beta)) + 1e-14
+
+
+def test_fseries_inverse_of_phihat_with_alternating_sign():
+    ns, beta = O.kernel_params(1e-6, 2.0)
+    for n in (20, 64, 91):
+        fs = O.fseries(n, ns, beta)
+        assert len(fs) == n // 2 + 1
+        for k in range(n // 2 + 1):
+            ph = O.phihat(2 * math.pi * k / n, ns, beta)
+            assert fs[k] * ph * (-1) ** k == pytest.approx(1.0, rel=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# Fold
+# ---------------------------------------------------------------------------
+def test_fold_closed_forms():
+    pi = math.pi
+    xs = [-pi, pi, 0.0, 20.0, -17.5, 3 * pi, -5 * pi + 0.25, 1e4, -1e4]
+    u = O.fold(xs)
+    assert u[0] == 0.0 and u[1] == 0.0 and u[2] == 0.5
+    for x, v in zip(xs, u):
+        assert 0.0 <= v < 1.0
+        # u = ((x + pi) / 2pi) mod 1, from mpmath at 50 digits
+        import mpmath
+        mpmath.mp.dps = 50
+        exact = mpmath.frac((mpmath.mpf(x) + mpmath.pi) / (2 * mpmath.pi))
+        assert abs(v - float(exact)) < 1e-15 * max(1.0, abs(x) / pi) or abs(abs(v - float(exact)) - 1) < 1e-12
+
+
+def test_fold_offset_and_scale():
+    x = np.array([5.0, -3.0, 12.5])
+    assert np.allclose(O.fold(x, 2.0, 0.5), O.fold((x - 2.0) * 0.5))
+
+
+# ---------------------------------------------------------------------------
+# Type-3 sizing: src/contrib/es_kernel/util.hpp:39-105
+# ---------------------------------------------------------------------------
+def _is_smooth(v):
+    for p in (2, 3, 5):
+        while v % p == 0:
+            v //= p
+    return v == 1
+
+
+def test_next235even_brute_force():
+    for n in range(-2, 2000):
+        r = O.next235even(n)
+        if n <= 2:
+            assert r == 2
+            continue
+        assert r >= n and r % 2 == 0 and _is_smooth(r)
+        assert not any(_is_smooth(v) for v in range(n + (n % 2), r, 2))
+    assert O.next235even(524) == 540 and O.next235even(144) == 144
+
+
+def test_t3_grid_info_cfg4():
+    ns, _ = O.kernel_params(1e-6, 2.0)
+    gi = O.GridInfoT3.make(ns, 2.0, 0.1, [-1e4, -1e4, -50], [1e4, 1e4, 50],
+                           [-0.02, -0.02, -1], [0.02, 0.02, 1])
+    assert gi.spread_grid_size == [262, 262, 72]
+    assert gi.fft_grid_size == [540, 540, 144]
+    assert gi.input_offsets == [0, 0, 0] and gi.output_offsets == [0, 0, 0]
+    # recentering keeps an offset once |center| >= 0.1 * half extent
+    gi2 = O.GridInfoT3.make(ns, 2.0, 0.1, [10.0], [20.0], [1.0], [2.0])
+    assert gi2.input_offsets == [15.0] and gi2.input_half_extent == [5.0]
+    assert gi2.output_offsets == [1.5] and gi2.output_half_extent == [0.5]
+    h, g = O.set_nhg(0.5, 5.0, 2.0, gi2.spread_grid_size[0])
+    assert h == pytest.approx(2 * math.pi / gi2.spread_grid_size[0])
+    assert g == pytest.approx(gi2.spread_grid_size[0] / (2 * 2.0 * 0.5))
+
+
+# ---------------------------------------------------------------------------
+# Direct sums vs brute-force numpy and the golden fixtures
+# ---------------------------------------------------------------------------
+def _brute(dim, modes, x, sign):
+    axes = [np.arange(N) - N // 2 for N in modes]
+    mesh = np.meshgrid(*axes, indexing="ij")
+    k = np.stack([m.ravel(order="F") for m in mesh], axis=1)
+    return k, np.exp(1j * sign * (k @ x))
+
+
+@pytest.mark.parametrize("dim,modes", [(1, (9,)), (2, (6, 5)), (3, (4, 3, 5))])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_direct_sums_vs_brute_force(dim, modes, sign):
+    rng = np.random.default_rng(3 + dim)
+    m = 31
+    x = rng.uniform(-7, 9, (dim, m))
+    c = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    k, E = _brute(dim, modes, x, sign)
+    kk = np.zeros((k.shape[0], 3), dtype=np.int64)
+    kk[:, :dim] = k
+    assert np.allclose(O.direct_type1(list(x), c, kk, sign), E @ c, rtol=1e-12, atol=1e-12)
+    f = rng.standard_normal(k.shape[0]) + 1j * rng.standard_normal(k.shape[0])
+    assert np.allclose(O.direct_type2(list(x), f, modes, sign), E.T @ f, rtol=1e-12, atol=1e-12)
+    s = rng.uniform(-3, 3, (dim, 11))
+    assert np.allclose(O.direct_type3(list(x), c, list(s), sign),
+                       np.exp(1j * sign * (s.T @ x)) @ c, rtol=1e-12, atol=1e-12)
+
+
+GOLD = np.load(os.path.join(HERE, "golden", "direct_small.npz"))
+GOLD_KEYS = sorted({k.rsplit("_", 1)[0] for k in GOLD.files})
+
+
+@pytest.mark.parametrize("key", GOLD_KEYS)
+def test_oracle_against_golden_fixtures(key):
+    g = {n: GOLD[f"{key}_{n}"] for n in ("x", "c", "modes", "t1", "f", "t2", "s", "t3")}
+    dim = g["x"].shape[0]
+    sign = 1 if key.endswith("sp") else -1
+    modes = tuple(int(v) for v in g["modes"])
+    x = list(g["x"])
+    kk = O.mode_vectors(modes, np.arange(int(np.prod(modes))))
+    assert O.rel_l2(O.direct_type1(x, g["c"], kk, sign), g["t1"]) < 1e-13
+    assert O.rel_l2(O.direct_type2(x, g["f"], modes, sign), g["t2"]) < 1e-13
+    assert O.rel_l2(O.direct_type3(x, g["c"], list(g["s"]), sign), g["t3"]) < 1e-13
+    # the NUFFT restatement reaches the requested tolerance on the same fixtures
+    for tol in (1e-4, 1e-9):
+        assert O.rel_l2(O.nufft1(x, g["c"], modes, sign, tol), g["t1"]) < 10 * tol
+        assert O.rel_l2(O.nufft2(x, g["f"], modes, sign, tol), g["t2"]) < 10 * tol
+        assert O.rel_l2(O.nufft3(x, g["c"], list(g["s"]), sign, tol), g["t3"]) < 10 * tol
+
+
+# ---------------------------------------------------------------------------
+# Pipeline vs the direct sum over tolerances, dimensions, upsampling
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,modes", [(1, (45,)), (2, (16, 23)), (3, (12, 10, 9))])
+@pytest.mark.parametrize("tol", [1e-3, 1e-6, 1e-10])
+@pytest.mark.parametrize("sigma", [2.0, 1.25])
+def test_pipeline_meets_tolerance(dim, modes, tol, sigma):
+    # The kernel width is clamped to 16 (es_kernel_param.hpp:36-37); at
+    # sigma = 1.25 that caps the reachable accuracy near exp(-16 pi sqrt(1-1/sigma)).
+    req = tol
+    ns, _ = O.kernel_params(req, sigma)
+    if sigma != 2.0:
+        tol = max(req, math.exp(-ns * math.pi * math.sqrt(1 - 1 / sigma)))
+    rng = np.random.default_rng(int(1e6 * req) + dim)
+    m = 200
+    lo_hi = [(20, 30), (-20, 20), (-20, -15)]
+    x = [rng.uniform(*lo_hi[d], m) for d in range(dim)]
+    c = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    kk = O.mode_vectors(modes, np.arange(int(np.prod(modes))))
+    assert O.rel_l2(O.nufft1(x, c, modes, -1, req, sigma), O.direct_type1(x, c, kk, -1)) < 20 * tol
+    f = rng.standard_normal(len(kk)) + 1j * rng.standard_normal(len(kk))
+    assert O.rel_l2(O.nufft2(x, f, modes, 1, req, sigma), O.direct_type2(x, f, modes, 1)) < 20 * tol
+    s = [rng.uniform(-30, 40, 77) for _ in range(dim)]
+    # type 3 applies the kernel twice (spread, then interpolate)
+    assert O.rel_l2(O.nufft3(x, c, s, -1, req, sigma), O.direct_type3(x, c, s, -1)) < 40 * tol
+
+
+def test_fft_order_equals_cmcl_rolled():
+    rng = np.random.default_rng(5)
+    modes = (7, 6)
+    x = [rng.uniform(-3, 3, 50) for _ in range(2)]
+    c = rng.standard_normal(50) + 1j * rng.standard_normal(50)
+    a = O.nufft1(x, c, modes, -1, 1e-8).reshape(6, 7)
+    b = O.nufft1(x, c, modes, -1, 1e-8, order_fft=True).reshape(6, 7)
+    # FFT order: index 0 <-> k = 0; CMCL: index 0 <-> k = -N/2
+    assert np.allclose(np.roll(a, (-(6 // 2), -(7 // 2)), axis=(0, 1)), b, atol=1e-12)
+
+
+def test_up_down_sample_round_trip_and_strides():
+    rng = np.random.default_rng(9)
+    for modes, n in [((7, 6), (16, 12)), ((8, 5, 3), (20, 10, 6))]:
+        dim = len(modes)
+        ones = [np.ones(v // 2 + 1) for v in n]
+        f = rng.standard_normal(int(np.prod(modes))) + 1j * rng.standard_normal(int(np.prod(modes)))
+        for order in (False, True):
+            grid = O.upsample(n, modes, ones, order, f)
+            assert np.count_nonzero(grid) == f.size  # zero padding elsewhere
+            back = O.downsample(n, modes, ones, order, grid)
+            assert np.allclose(back, f, atol=0)
+        # strided output: every other element, contiguous otherwise
+        st = [2 * s for s in O.contiguous_strides(modes)]
+        out = O.downsample(n, modes, ones, False, O.upsample(n, modes, ones, False, f), out_strides=st)
+        assert np.allclose(out[::2], f, atol=0) and np.all(out[1::2] == 0)
+
+
+def test_spread_single_point_wraps_periodically():
+    ns, beta = O.kernel_params(1e-6, 2.0)
+    n = [20, 14]
+    u = [np.array([0.99]), np.array([0.01])]  # near the high and low edges
+    grid = O.spread(n, ns, beta, u, np.array([1.0 + 0j]))
+    # all weight sits within ns cells of the point, wrapped around both edges
+    ix = {(i0) % n[0] for i0 in range(int(np.ceil(0.99 * 20 - ns / 2)), int(np.ceil(0.99 * 20 - ns / 2)) + ns)}
+    iy = {(i1) % n[1] for i1 in range(int(np.ceil(0.01 * 14 - ns / 2)), int(np.ceil(0.01 * 14 - ns / 2)) + ns)}
+    nz = {(int(b), int(a)) for b, a in zip(*np.nonzero(grid))}
+    assert nz <= {(b, a) for a in ix for b in iy}
+    # total mass = (sum_j phi_j)^2 for the two axes
+    def mass(v, nn):
+        xi = v * nn
+        i0 = math.ceil(xi - ns / 2)
+        return sum(O.es_kernel([i0 + j - xi], ns, beta)[0] for j in range(ns))
+    assert grid.sum().real == pytest.approx(mass(0.99, 20) * mass(0.01, 14), rel=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# Planted mistakes: the checks above must catch them
+# ---------------------------------------------------------------------------
+def _t1_case():
+    rng = np.random.default_rng(77)
+    modes = (16, 12)
+    x = [rng.uniform(-3, 3, 300) for _ in range(2)]
+    c = rng.standard_normal(300) + 1j * rng.standard_normal(300)
+    kk = O.mode_vectors(modes, np.arange(int(np.prod(modes))))
+    return x, c, modes, O.direct_type1(x, c, kk, -1)
+
+
+def test_planted_wrong_sign_is_caught():
+    x, c, modes, ref = _t1_case()
+    assert O.rel_l2(O.nufft1(x, c, modes, +1, 1e-8), ref) > 0.5
+
+
+def test_planted_missing_alternating_sign_is_caught():
+    x, c, modes, ref = _t1_case()
+    ns, beta = O.kernel_params(1e-8, 2.0)
+    n = O.fine_grid_size(modes, ns, 2.0)
+    grid = O.spread(n, ns, beta, [O.fold(v) for v in x], c)
+    F = O.fft_grid(grid, -1)
+    fs = [np.abs(O.fseries(v, ns, beta)) for v in n]  # (-1)^k dropped
+    assert O.rel_l2(O.downsample(n, modes, fs, False, F), ref) > 0.5
+
+
+def test_planted_transposed_axes_are_caught():
+    x, c, modes, ref = _t1_case()
+    got = O.nufft1([x[1], x[0]], c, (modes[1], modes[0]), -1, 1e-8)
+    assert O.rel_l2(got, ref) > 0.5
