@@ -94,3 +94,23 @@ def test_t3_invalid_bounds_is_input_error():
         nnf.PlanT3.grid_memory_size(opt, [1.0], [0.0], [0.0], [1.0])
     with pytest.raises(nnf.InputError):
         nnf.PlanT3.grid_memory_size(opt, [0.0], [1.0], [2.0], [1.0])
+
+
+def test_cpp_dropin_example_compiles_and_links(tmp_path):
+    """A program written against the reference's C++ GPU API builds against
+    include/neonufft and links against libneonufft_amd.so."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no host C++ compiler")
+    exe = tmp_path / "dropin"
+    cmd = [cxx, "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", "-I/opt/rocm/include",
+           "-D__HIP_PLATFORM_AMD__", os.path.join(ROOT, "tests", "cpp", "dropin_example.cpp"),
+           "-o", str(exe), f"-L{os.path.dirname(L.LIB_PATH)}", "-lneonufft_amd",
+           f"-Wl,-rpath,{os.path.dirname(L.LIB_PATH)}", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True)  # no arguments: no GPU calls
+    assert r.returncode == 0 and "usage" in r.stdout
