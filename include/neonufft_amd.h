@@ -51,6 +51,7 @@ typedef struct nnf_options {
 
 typedef struct nnf_plan_s* nnf_plan;
 typedef struct nnf_plan_t3_s* nnf_plan_t3;
+typedef struct nnf_batch_s* nnf_batch;
 
 /* Fills the reference defaults: tol 1e-3, upsampfac 2, recenter 0.1, CMCL. */
 void nnf_options_default(nnf_options* opt);
@@ -174,6 +175,43 @@ int nnf_plan_interp(nnf_plan plan, void* out);
 int nnf_plan_fine_grid(nnf_plan plan, void** grid);
 /* Device pointer of the deconvolution factors fs_d[0..n_d/2] (T). */
 int nnf_plan_fseries(nnf_plan plan, int d, const void** fs);
+
+/* ---- Batched uneven transforms (no reference counterpart) ----------------
+ * `count` independent type 1 / type 2 transforms of one precision, dimension
+ * and sign, each with its own points and mode counts, issued together: one
+ * zero fill, one spreading / interpolation launch over all of their
+ * subproblems, one batched FFT per distinct mode shape. Replaces a loop over
+ * `count` plans of the reference's gpu::Plan (include/neonufft/gpu/plan.hpp),
+ * with the same result per transform.
+ *   num_nu[t]            points of transform t
+ *   loc[t * dim + d]     device coordinates of transform t, dimension d
+ *   modes[t * dim + d]   N_d of transform t
+ *   use_graph            non-zero: capture each transform type once into a
+ *                        HIP graph on first use and replay it afterwards.
+ * Mode arrays of every transform are contiguous, x fastest (strides
+ * 1, N_0, N_0 N_1). The pointer tables of transform_type_{1,2} are host
+ * arrays of `count` device pointers. */
+int nnf_batch_create(nnf_batch* batch, int precision, int dim, const nnf_options* opt, int sign,
+                     int64_t count, const int64_t* num_nu, const void* const* loc,
+                     const int64_t* modes, void* stream, int use_graph);
+int nnf_batch_set_points(nnf_batch batch, const int64_t* num_nu, const void* const* loc);
+int nnf_batch_transform_type_1(nnf_batch batch, const void* const* in, void* const* out);
+int nnf_batch_transform_type_2(nnf_batch batch, const void* const* in, void* const* out);
+
+typedef struct nnf_batch_info {
+  int precision;
+  int dim;
+  int n_spread;
+  int64_t count;
+  int64_t num_points;       /* sum over transforms */
+  int64_t num_subproblems;  /* one launch covers all of them */
+  int64_t num_shapes;       /* distinct mode shapes = batched FFT plans */
+  int64_t grid_cells;       /* fine-grid cells over all transforms */
+  int graph;                /* 1 once a captured graph is being replayed */
+} nnf_batch_info;
+
+int nnf_batch_get_info(nnf_batch batch, nnf_batch_info* info);
+int nnf_batch_destroy(nnf_batch batch);
 
 #ifdef __cplusplus
 }

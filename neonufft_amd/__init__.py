@@ -18,7 +18,7 @@ from typing import Optional, Sequence, Union
 from . import _lib as _L
 
 __all__ = [
-    "Options", "ModeOrder", "Plan", "PlanT3", "GenericError", "InputError", "InternalError",
+    "Options", "ModeOrder", "Plan", "PlanT3", "BatchPlan", "GenericError", "InputError", "InternalError",
     "MemoryAllocError", "NotImplementedError_", "GPUError", "UnknownError", "version",
 ]
 
@@ -316,6 +316,86 @@ class PlanT3:
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
             _check(self._L.nnf_plan_t3_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchPlan:
+    """Many independent type 1 / type 2 transforms of uneven sizes, issued
+    together (extension; no reference counterpart).
+
+    Equivalent to ``count`` plans ``Plan(opt, sign, num_nu[t], loc[t],
+    modes[t])`` run one after the other, but each transform call makes one
+    spreading or interpolation launch over every transform's subproblems and
+    one batched FFT per distinct mode shape; with ``use_graph`` the sequence
+    is captured once into a HIP graph and replayed. Mode arrays are contiguous
+    with x fastest.
+    """
+
+    def __init__(self, opt: Options, sign: int, num_nu: Sequence[int],
+                 loc: Sequence[Sequence[Pointer]], modes: Sequence[Sequence[int]],
+                 stream=None, dtype=None, use_graph: bool = True):
+        self._L = _L.lib()
+        self.count = len(modes)
+        if self.count == 0 or len(num_nu) != self.count or len(loc) != self.count:
+            raise InputError("num_nu, loc and modes need one entry per transform")
+        self.dim = len(modes[0])
+        if any(len(m) != self.dim for m in modes) or any(len(l) != self.dim for l in loc):
+            raise InputError("every transform needs the same dimension")
+        self.modes = [[int(v) for v in m] for m in modes]
+        self.precision = _dtype_code(dtype) if dtype is not None else _precision_of(loc[0][0])
+        self.num_nu = [int(v) for v in num_nu]
+        self._h = ctypes.c_void_p()
+        flat_loc = [_ptr(p) for l in loc for p in l]
+        vpp, _keep = _vpp(flat_loc)
+        nn = (ctypes.c_int64 * self.count)(*self.num_nu)
+        md = (ctypes.c_int64 * (self.count * self.dim))(*[v for m in self.modes for v in m])
+        _check(self._L.nnf_batch_create(ctypes.byref(self._h), self.precision, self.dim,
+                                        ctypes.byref(opt._c()), int(sign), self.count,
+                                        nn, vpp, md, ctypes.c_void_p(_stream(stream)),
+                                        int(bool(use_graph))))
+
+    def set_points(self, num_nu: Sequence[int], loc: Sequence[Sequence[Pointer]]) -> None:
+        if len(num_nu) != self.count or len(loc) != self.count:
+            raise InputError("num_nu and loc need one entry per transform")
+        vpp, _keep = _vpp([_ptr(p) for l in loc for p in l])
+        nn = (ctypes.c_int64 * self.count)(*[int(v) for v in num_nu])
+        _check(self._L.nnf_batch_set_points(self._h, nn, vpp))
+        self.num_nu = [int(v) for v in num_nu]
+
+    def _tables(self, ins, outs):
+        if len(ins) != self.count or len(outs) != self.count:
+            raise InputError("one input and one output per transform")
+        a, ka = _vpp([_ptr(p) for p in ins])
+        b, kb = _vpp([_ptr(p) for p in outs])
+        return a, b, (ka, kb)
+
+    def transform_type_1(self, ins: Sequence[Pointer], outs: Sequence[Pointer]) -> None:
+        a, b, _keep = self._tables(ins, outs)
+        _check(self._L.nnf_batch_transform_type_1(self._h, a, b))
+
+    def transform_type_2(self, ins: Sequence[Pointer], outs: Sequence[Pointer]) -> None:
+        a, b, _keep = self._tables(ins, outs)
+        _check(self._L.nnf_batch_transform_type_2(self._h, a, b))
+
+    def info(self) -> dict:
+        i = _L.NnfBatchInfo()
+        _check(self._L.nnf_batch_get_info(self._h, ctypes.byref(i)))
+        return {
+            "precision": "float32" if i.precision == 0 else "float64",
+            "dim": i.dim, "n_spread": i.n_spread, "count": i.count,
+            "num_points": i.num_points, "num_subproblems": i.num_subproblems,
+            "num_shapes": i.num_shapes, "grid_cells": i.grid_cells, "graph": bool(i.graph),
+        }
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _check(self._L.nnf_batch_destroy(self._h))
             self._h = ctypes.c_void_p()
 
     def __del__(self):

@@ -17,6 +17,8 @@ EXPORTED = (
     "nnf_plan_t3_transform", "nnf_plan_t3_reset", "nnf_plan_t3_destroy",
     "nnf_plan_get_info", "nnf_plan_t3_get_info", "nnf_plan_spread", "nnf_plan_interp",
     "nnf_plan_fine_grid", "nnf_plan_fseries", "nnf_plan_create_tuned",
+    "nnf_batch_create", "nnf_batch_set_points", "nnf_batch_transform_type_1",
+    "nnf_batch_transform_type_2", "nnf_batch_get_info", "nnf_batch_destroy",
 )
 
 
@@ -58,6 +60,20 @@ class NnfPlanInfo(ctypes.Structure):
         ("num_out_points", ctypes.c_int64),
         ("horner_degree", ctypes.c_int),
         ("quad_nodes", ctypes.c_int),
+    ]
+
+
+class NnfBatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int),
+        ("dim", ctypes.c_int),
+        ("n_spread", ctypes.c_int),
+        ("count", ctypes.c_int64),
+        ("num_points", ctypes.c_int64),
+        ("num_subproblems", ctypes.c_int64),
+        ("num_shapes", ctypes.c_int64),
+        ("grid_cells", ctypes.c_int64),
+        ("graph", ctypes.c_int),
     ]
 
 
@@ -110,6 +126,13 @@ def lib():
     L.nnf_plan_interp.argtypes = [vp, vp]
     L.nnf_plan_fine_grid.argtypes = [vp, vpp]
     L.nnf_plan_fseries.argtypes = [vp, c_int, vpp]
+    L.nnf_batch_create.argtypes = [vpp, c_int, c_int, optp, c_int, i64, i64p, vpp, i64p, vp,
+                                   c_int]
+    L.nnf_batch_set_points.argtypes = [vp, i64p, vpp]
+    L.nnf_batch_transform_type_1.argtypes = [vp, vpp, vpp]
+    L.nnf_batch_transform_type_2.argtypes = [vp, vpp, vpp]
+    L.nnf_batch_get_info.argtypes = [vp, ctypes.POINTER(NnfBatchInfo)]
+    L.nnf_batch_destroy.argtypes = [vp]
     for name in EXPORTED:
         if name not in ("nnf_last_error", "nnf_version", "nnf_options_default"):
             getattr(L, name).restype = ctypes.c_int

@@ -1,0 +1,365 @@
+// Many independent type 1 / type 2 transforms of uneven sizes in one plan.
+//
+// No reference counterpart: the reference runs one gpu::Plan per transform
+// (src/neonufft/gpu/plan_impl.hpp:36-167). Here every transform keeps its own
+// points and fine grid, but the work of all of them is issued together:
+//  - one zero fill over all fine grids,
+//  - one spreading (or interpolation) launch over the union of all
+//    subproblems, each subproblem naming its transform (Subprob::job),
+//  - one batched rocFFT plan per distinct grid shape (grids of one shape are
+//    contiguous),
+//  - one batched deconvolution launch per shape.
+// The sequence can be captured once into a HIP graph and replayed; per-call
+// input / output pointers live in device-side tables refreshed before each
+// replay, so the graph stays valid when the caller's buffers change.
+#pragma once
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "plan_impl.hpp"
+
+namespace nnf {
+
+// Stream-ordered upload of a small host table into a fixed device buffer.
+// Unchanged contents are not sent again; otherwise the bytes go through one
+// of a few pinned slots, each reused only after its previous copy finished.
+class TableUpload {
+public:
+  TableUpload() = default;
+  TableUpload(const TableUpload&) = delete;
+  TableUpload& operator=(const TableUpload&) = delete;
+  ~TableUpload() {
+    for (int k = 0; k < kSlots; ++k) {
+      if (ev_[k]) (void)hipEventDestroy(ev_[k]);
+      if (slot_[k]) (void)hipHostFree(slot_[k]);
+    }
+  }
+
+  void init(std::size_t bytes) {
+    bytes_ = bytes;
+    for (int k = 0; k < kSlots; ++k) {
+      check_hip(hipHostMalloc(&slot_[k], bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+      check_hip(hipEventCreateWithFlags(&ev_[k], hipEventDisableTiming), "hipEventCreate");
+    }
+    last_.assign(bytes, 0);
+    valid_ = false;
+  }
+
+  void invalidate() { valid_ = false; }
+
+  void upload(void* dst, const void* src, hipStream_t stream) {
+    if (valid_ && std::memcmp(last_.data(), src, bytes_) == 0) return;
+    const int k = next_;
+    next_ = (next_ + 1) % kSlots;
+    if (used_[k]) check_hip(hipEventSynchronize(ev_[k]), "hipEventSynchronize");
+    std::memcpy(slot_[k], src, bytes_);
+    check_hip(hipMemcpyAsync(dst, slot_[k], bytes_, hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync(table)");
+    check_hip(hipEventRecord(ev_[k], stream), "hipEventRecord");
+    used_[k] = true;
+    std::memcpy(last_.data(), src, bytes_);
+    valid_ = true;
+  }
+
+private:
+  static constexpr int kSlots = 4;
+  std::size_t bytes_ = 0;
+  void* slot_[kSlots] = {};
+  hipEvent_t ev_[kSlots] = {};
+  bool used_[kSlots] = {};
+  int next_ = 0;
+  bool valid_ = false;
+  std::vector<unsigned char> last_;
+};
+
+template <typename T, int DIM>
+class BatchPlanImpl {
+public:
+  BatchPlanImpl(const neonufft::Options& opt, int sign, IntType count, const IntType* num_nu,
+                const T* const* loc, const IntType* modes, hipStream_t stream,
+                std::shared_ptr<neonufft::Allocator> alloc, bool use_graph,
+                const PlanTuning& tune = PlanTuning())
+      : opt_(opt), sign_(sign), stream_(stream), alloc_(std::move(alloc)), tune_(tune),
+        use_graph_(use_graph) {
+    if (!alloc_) throw neonufft::InputError("Neonufft:: null allocator");
+    if (count <= 0) throw neonufft::InputError("Neonufft:: batch count must be positive");
+    if (!num_nu || !modes) throw neonufft::InputError("Neonufft:: null argument");
+    ks_ = make_kernel_setup(opt_.tol, opt_.upsampfac, std::is_same_v<T, float>);
+    count_ = static_cast<int>(count);
+    items_.resize(count_);
+    // group transforms by mode shape (equal modes -> equal fine grid)
+    std::map<std::vector<IntType>, int> shape_to_group;
+    for (int t = 0; t < count_; ++t) {
+      Item& it = items_[t];
+      std::vector<IntType> key;
+      for (int d = 0; d < 3; ++d) {
+        it.modes[d] = 1;
+        it.n[d] = 1;
+      }
+      for (int d = 0; d < DIM; ++d) {
+        const IntType N = modes[t * DIM + d];
+        if (N <= 0) throw neonufft::InputError("Neonufft:: modes must be positive");
+        it.modes[d] = N;
+        it.n[d] = std::max<IntType>(2 * ks_.ns,
+                                    static_cast<IntType>(static_cast<double>(N) * opt_.upsampfac));
+        key.push_back(N);
+      }
+      auto f = shape_to_group.find(key);
+      if (f == shape_to_group.end()) {
+        Group g;
+        for (int d = 0; d < 3; ++d) {
+          g.modes[d] = it.modes[d];
+          g.n[d] = it.n[d];
+        }
+        groups_.push_back(std::move(g));
+        f = shape_to_group.emplace(key, static_cast<int>(groups_.size()) - 1).first;
+      }
+      it.group = f->second;
+      groups_[it.group].members.push_back(t);
+    }
+    // lay the grids out group by group
+    std::size_t off = 0;
+    for (Group& g : groups_) {
+      g.cells = 1;
+      for (int d = 0; d < DIM; ++d) g.cells *= static_cast<std::size_t>(g.n[d]);
+      g.offset = off;
+      for (std::size_t k = 0; k < g.members.size(); ++k) {
+        items_[g.members[k]].grid_off = off + k * g.cells;
+      }
+      off += g.cells * g.members.size();
+    }
+    grids_.reset(off, alloc_);
+    for (Group& g : groups_) {
+      g.fft = FftPlan(std::is_same_v<T, double>, DIM, g.n, sign_,
+                      static_cast<IntType>(g.members.size()), stream_, alloc_);
+      for (int d = 0; d < DIM; ++d) {
+        g.fs[d].reset(static_cast<std::size_t>(g.n[d] / 2 + 1), alloc_);
+        launch_fseries<T>(g.fs[d].data(), g.n[d], ks_.quad, stream_);
+      }
+      g.geom = choose_bins(DIM, g.n, ks_.ns, sizeof(Cplx<T>), tune_.bin);
+      max_lds_ = std::max(max_lds_, tile_bytes<T>(g.geom));
+    }
+    std::size_t io_off = 0;
+    for (Group& g : groups_) {
+      g.io_off = io_off;
+      io_off += g.members.size();
+    }
+    pts_.resize(count_);
+    sjobs_.reset(count_, alloc_);
+    ijobs_.reset(count_, alloc_);
+    io_.reset(count_, alloc_);
+    host_sjobs_.resize(count_);
+    host_ijobs_.resize(count_);
+    host_io_.resize(count_);
+    up_sjobs_.init(sizeof(SpreadJob<T, DIM>) * count_);
+    up_ijobs_.init(sizeof(InterpJob<T, DIM>) * count_);
+    up_io_.init(sizeof(void*) * count_);
+    check_hip(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    set_points(num_nu, loc);
+  }
+
+  ~BatchPlanImpl() {
+    drop_graphs();
+    if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
+  }
+
+  BatchPlanImpl(const BatchPlanImpl&) = delete;
+  BatchPlanImpl& operator=(const BatchPlanImpl&) = delete;
+
+  // loc: count * DIM device pointers, transform-major.
+  void set_points(const IntType* num_nu, const T* const* loc) {
+    if (!num_nu) throw neonufft::InputError("Neonufft:: null point counts");
+    drop_graphs();
+    std::size_t total_sub = 0;
+    total_points_ = 0;
+    for (int t = 0; t < count_; ++t) {
+      if (num_nu[t] < 0) throw neonufft::InputError("Neonufft:: negative number of points");
+      Loc<T> l{{nullptr, nullptr, nullptr}};
+      for (int d = 0; d < DIM; ++d) l.p[d] = loc ? loc[t * DIM + d] : nullptr;
+      FoldParams fp{{0, 0, 0}, {1, 1, 1}};
+      pts_[t].build(l, static_cast<std::size_t>(num_nu[t]), fp, groups_[items_[t].group].geom,
+                    ks_.ns, tune_.max_points, stream_, alloc_);
+      items_[t].sub_off = total_sub;
+      total_sub += pts_[t].nsub();
+      total_points_ += static_cast<std::size_t>(num_nu[t]);
+    }
+    subs_.ensure(total_sub ? total_sub : 1, alloc_);
+    up_sjobs_.invalidate();
+    up_ijobs_.invalidate();
+    nsub_ = total_sub;
+    for (int t = 0; t < count_; ++t) {
+      launch_subprob_retag(pts_[t].subs(), pts_[t].nsub(), static_cast<uint32_t>(t),
+                           subs_.data() + items_[t].sub_off, stream_);
+    }
+    check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+
+  // in / out: count device pointers each; modes contiguous (x fastest).
+  void transform_type_1(const Cplx<T>* const* in, Cplx<T>* const* out) {
+    if (!in || !out) throw neonufft::InputError("Neonufft:: null pointer table");
+    for (int t = 0; t < count_; ++t) {
+      const Item& it = items_[t];
+      host_sjobs_[t] = SpreadJob<T, DIM>{pts_[t].recs(), pts_[t].perm(), in[t], nullptr,
+                                         grids_.data() + it.grid_off, groups_[it.group].geom};
+    }
+    up_sjobs_.upload(sjobs_.data(), host_sjobs_.data(), stream_);
+    upload_io(reinterpret_cast<void* const*>(out));
+    run(1);
+  }
+
+  void transform_type_2(const Cplx<T>* const* in, Cplx<T>* const* out) {
+    if (!in || !out) throw neonufft::InputError("Neonufft:: null pointer table");
+    for (int t = 0; t < count_; ++t) {
+      const Item& it = items_[t];
+      host_ijobs_[t] = InterpJob<T, DIM>{pts_[t].recs(), pts_[t].perm(),
+                                         grids_.data() + it.grid_off, nullptr, out[t],
+                                         groups_[it.group].geom};
+    }
+    up_ijobs_.upload(ijobs_.data(), host_ijobs_.data(), stream_);
+    upload_io(const_cast<void* const*>(reinterpret_cast<const void* const*>(in)));
+    run(2);
+  }
+
+  std::size_t total_points() const { return total_points_; }
+  std::size_t num_subproblems() const { return nsub_; }
+  int num_groups() const { return static_cast<int>(groups_.size()); }
+  std::size_t grid_cells() const { return grids_.size(); }
+  int count() const { return count_; }
+  bool graph_active() const { return graph_ok_[0] || graph_ok_[1]; }
+  const KernelSetup& kernel() const { return ks_; }
+
+private:
+  struct Item {
+    IntType modes[3];
+    IntType n[3];
+    std::size_t grid_off = 0;
+    std::size_t sub_off = 0;
+    int group = 0;
+  };
+  struct Group {
+    IntType modes[3];
+    IntType n[3];
+    std::size_t cells = 0, offset = 0;
+    std::vector<int> members;
+    FftPlan fft;
+    DeviceBuffer<T> fs[3];
+    BinGeom geom{};
+    std::size_t io_off = 0;  // first entry of this group's caller pointers in io_
+  };
+
+  int order_fft() const { return opt_.order == NEONUFFT_MODE_ORDER_FFT ? 1 : 0; }
+
+  // Caller pointers in group-major order: group g reads io_[g.io_off ...].
+  void upload_io(void* const* ptrs) {
+    std::size_t k = 0;
+    for (const Group& g : groups_) {
+      for (int t : g.members) host_io_[k++] = ptrs[t];
+    }
+    up_io_.upload(io_.data(), host_io_.data(), stream_);
+  }
+
+  void enqueue(int type, hipStream_t s) {
+    if (type == 1) {
+      launch_fill_zero<T>(grids_.data(), grids_.size(), s);
+      launch_spread_batch<T, DIM>(ks_.ns, sjobs_.data(), subs_.data(), nsub_, max_lds_, ks_.es, s);
+      for (Group& g : groups_) {
+        g.fft.set_stream(s);
+        g.fft.execute(grids_.data() + g.offset);
+        T* fs[3] = {g.fs[0].data(), g.fs[1].data(), g.fs[2].data()};
+        IntType st[3] = {1, g.modes[0], g.modes[0] * g.modes[1]};
+        launch_downsample_batch<T, DIM>(order_fft(), grids_.data() + g.offset, g.cells, g.n, fs,
+                                        reinterpret_cast<Cplx<T>* const*>(io_.data() + g.io_off), st,
+                                        g.modes, static_cast<int>(g.members.size()), s);
+      }
+    } else {
+      for (Group& g : groups_) {
+        T* fs[3] = {g.fs[0].data(), g.fs[1].data(), g.fs[2].data()};
+        IntType st[3] = {1, g.modes[0], g.modes[0] * g.modes[1]};
+        launch_upsample_batch<T, DIM>(order_fft(),
+                                      reinterpret_cast<const Cplx<T>* const*>(io_.data() + g.io_off), st,
+                                      g.modes, fs, grids_.data() + g.offset, g.cells, g.n,
+                                      static_cast<int>(g.members.size()), s);
+        g.fft.set_stream(s);
+        g.fft.execute(grids_.data() + g.offset);
+      }
+      launch_interp_batch<T, DIM>(ks_.ns, ijobs_.data(), subs_.data(), nsub_, max_lds_, ks_.es, s);
+    }
+  }
+
+  void run(int type) {
+    const int k = type - 1;
+    if (use_graph_ && !graph_failed_[k] && !graph_ok_[k]) capture(type);
+    if (graph_ok_[k]) {
+      check_hip(hipGraphLaunch(exec_[k], stream_), "hipGraphLaunch");
+      return;
+    }
+    enqueue(type, stream_);
+  }
+
+  void capture(int type) {
+    const int k = type - 1;
+    hipGraph_t graph = nullptr;
+    // the capture runs on a private stream; the replay on the plan's stream
+    check_hip(hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeThreadLocal),
+              "hipStreamBeginCapture");
+    bool ok = true;
+    try {
+      enqueue(type, capture_stream_);
+    } catch (...) {
+      ok = false;
+    }
+    const hipError_t end = hipStreamEndCapture(capture_stream_, &graph);
+    (void)hipGetLastError();
+    for (Group& g : groups_) g.fft.set_stream(stream_);
+    if (ok && end == hipSuccess && graph) {
+      if (hipGraphInstantiate(&exec_[k], graph, nullptr, nullptr, 0) == hipSuccess) {
+        graph_ok_[k] = true;
+      } else {
+        (void)hipGetLastError();
+        exec_[k] = nullptr;
+      }
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!graph_ok_[k]) graph_failed_[k] = true;
+  }
+
+  void drop_graphs() {
+    for (int k = 0; k < 2; ++k) {
+      if (exec_[k]) (void)hipGraphExecDestroy(exec_[k]);
+      exec_[k] = nullptr;
+      graph_ok_[k] = false;
+      graph_failed_[k] = false;
+    }
+  }
+
+  neonufft::Options opt_;
+  int sign_;
+  hipStream_t stream_;
+  std::shared_ptr<neonufft::Allocator> alloc_;
+  PlanTuning tune_;
+  bool use_graph_;
+  KernelSetup ks_;
+  int count_ = 0;
+  std::vector<Item> items_;
+  std::vector<Group> groups_;
+  DeviceBuffer<Cplx<T>> grids_;
+  std::vector<PointSet<T, DIM>> pts_;
+  DeviceBuffer<Subprob> subs_;
+  std::size_t nsub_ = 0, total_points_ = 0, max_lds_ = 0;
+  DeviceBuffer<SpreadJob<T, DIM>> sjobs_;
+  DeviceBuffer<InterpJob<T, DIM>> ijobs_;
+  DeviceBuffer<void*> io_;  // outputs of type 1, inputs of type 2
+  std::vector<SpreadJob<T, DIM>> host_sjobs_;
+  std::vector<InterpJob<T, DIM>> host_ijobs_;
+  std::vector<void*> host_io_;
+  TableUpload up_sjobs_, up_ijobs_, up_io_;
+  hipStream_t capture_stream_ = nullptr;
+  hipGraphExec_t exec_[2] = {nullptr, nullptr};
+  bool graph_ok_[2] = {false, false};
+  bool graph_failed_[2] = {false, false};
+};
+
+}  // namespace nnf

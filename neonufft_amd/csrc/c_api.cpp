@@ -8,6 +8,7 @@
 #include <new>
 
 #include <neonufft_amd.h>
+#include "batch.hpp"
 #include "plan_impl.hpp"
 
 namespace {
@@ -92,6 +93,46 @@ struct PlanT3Holder final : PlanT3Iface {
   void info(nnf_plan_info* i) override { impl.info(i); }
 };
 
+struct BatchIface {
+  virtual ~BatchIface() = default;
+  virtual void set_points(const int64_t* n, const void* const* loc) = 0;
+  virtual void t1(const void* const* in, void* const* out) = 0;
+  virtual void t2(const void* const* in, void* const* out) = 0;
+  virtual void info(nnf_batch_info* i) = 0;
+};
+
+template <typename T, int DIM>
+struct BatchHolder final : BatchIface {
+  nnf::BatchPlanImpl<T, DIM> impl;
+  BatchHolder(const neonufft::Options& o, int sign, IntType count, const int64_t* num_nu,
+              const void* const* loc, const int64_t* modes, hipStream_t s,
+              std::shared_ptr<neonufft::Allocator> a, bool graph)
+      : impl(o, sign, count, num_nu, reinterpret_cast<const T* const*>(loc), modes, s,
+             std::move(a), graph) {}
+  void set_points(const int64_t* n, const void* const* loc) override {
+    impl.set_points(n, reinterpret_cast<const T* const*>(loc));
+  }
+  void t1(const void* const* in, void* const* out) override {
+    impl.transform_type_1(reinterpret_cast<const Cplx<T>* const*>(in),
+                          reinterpret_cast<Cplx<T>* const*>(out));
+  }
+  void t2(const void* const* in, void* const* out) override {
+    impl.transform_type_2(reinterpret_cast<const Cplx<T>* const*>(in),
+                          reinterpret_cast<Cplx<T>* const*>(out));
+  }
+  void info(nnf_batch_info* i) override {
+    i->precision = std::is_same_v<T, float> ? NNF_F32 : NNF_F64;
+    i->dim = DIM;
+    i->n_spread = impl.kernel().ns;
+    i->count = impl.count();
+    i->num_points = static_cast<int64_t>(impl.total_points());
+    i->num_subproblems = static_cast<int64_t>(impl.num_subproblems());
+    i->num_shapes = impl.num_groups();
+    i->grid_cells = static_cast<int64_t>(impl.grid_cells());
+    i->graph = impl.graph_active() ? 1 : 0;
+  }
+};
+
 neonufft::Options to_options(const nnf_options* o) {
   neonufft::Options opt;
   if (o) {
@@ -161,6 +202,10 @@ struct nnf_plan_s {
 };
 struct nnf_plan_t3_s {
   std::unique_ptr<PlanT3Iface> p;
+};
+
+struct nnf_batch_s {
+  std::unique_ptr<BatchIface> p;
 };
 
 #define NNF_REQUIRE_PLAN(pl)                                                  \
@@ -389,6 +434,58 @@ NEONUFFT_EXPORT int nnf_plan_fseries(nnf_plan plan, int d, const void** fs) {
     *fs = plan->p->fseries(d);
     if (!*fs) throw neonufft::InputError("Neonufft:: dimension out of range");
   });
+}
+
+NEONUFFT_EXPORT int nnf_batch_create(nnf_batch* batch, int precision, int dim,
+                                     const nnf_options* opt, int sign, int64_t count,
+                                     const int64_t* num_nu, const void* const* loc,
+                                     const int64_t* modes, void* stream, int use_graph) {
+  return guarded([&] {
+    if (!batch) throw neonufft::InputError("Neonufft:: null batch output pointer");
+    *batch = nullptr;
+    if (!modes || !num_nu) throw neonufft::InputError("Neonufft:: null modes or point counts");
+    auto h = std::make_unique<nnf_batch_s>();
+    h->p.reset(make_holder<BatchHolder, BatchIface>(
+        precision, dim, to_options(opt), sign, static_cast<IntType>(count), num_nu, loc, modes,
+        static_cast<hipStream_t>(stream), default_alloc(), use_graph != 0));
+    *batch = h.release();
+  });
+}
+
+NEONUFFT_EXPORT int nnf_batch_set_points(nnf_batch batch, const int64_t* num_nu,
+                                         const void* const* loc) {
+  return guarded([&] {
+    NNF_REQUIRE_PLAN(batch);
+    batch->p->set_points(num_nu, loc);
+  });
+}
+
+NEONUFFT_EXPORT int nnf_batch_transform_type_1(nnf_batch batch, const void* const* in,
+                                               void* const* out) {
+  return guarded([&] {
+    NNF_REQUIRE_PLAN(batch);
+    batch->p->t1(in, out);
+  });
+}
+
+NEONUFFT_EXPORT int nnf_batch_transform_type_2(nnf_batch batch, const void* const* in,
+                                               void* const* out) {
+  return guarded([&] {
+    NNF_REQUIRE_PLAN(batch);
+    batch->p->t2(in, out);
+  });
+}
+
+NEONUFFT_EXPORT int nnf_batch_get_info(nnf_batch batch, nnf_batch_info* info) {
+  return guarded([&] {
+    NNF_REQUIRE_PLAN(batch);
+    if (!info) throw neonufft::InputError("Neonufft:: null info");
+    batch->p->info(info);
+  });
+}
+
+NEONUFFT_EXPORT int nnf_batch_destroy(nnf_batch batch) {
+  return guarded([&] { delete batch; });
 }
 
 }  // extern "C"

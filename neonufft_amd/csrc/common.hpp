@@ -164,12 +164,13 @@ struct CellPack<1> {
   static constexpr int max_bin = 1 << 30;
 };
 
-// One unit of spreading/interpolation work: up to `max_points` points of one bin.
+// One unit of spreading/interpolation work: up to `max_points` points of one
+// bin; `job` selects the transform in batched launches (0 otherwise).
 struct Subprob {
   uint32_t bin;
   uint32_t begin;
   uint32_t count;
-  uint32_t pad;
+  uint32_t job;
 };
 
 // Geometry of a grid and its bin (subgrid tile) decomposition.

@@ -73,6 +73,11 @@ void FftPlan::destroy() noexcept {
   work_.release();
 }
 
+void FftPlan::set_stream(hipStream_t stream) {
+  if (!info_) throw neonufft::InternalError("Neonufft:: FFT plan not initialised");
+  check_fft(rocfft_execution_info_set_stream(info_, stream), "rocfft_execution_info_set_stream");
+}
+
 void FftPlan::execute(void* data) const {
   if (!plan_) throw neonufft::InternalError("Neonufft:: FFT plan not initialised");
   void* buffers[1] = {data};

@@ -28,6 +28,8 @@ public:
 
   void execute(void* data) const;
   bool valid() const { return plan_ != nullptr; }
+  // Changes the stream later executions are enqueued on.
+  void set_stream(hipStream_t stream);
 
 private:
   void destroy() noexcept;

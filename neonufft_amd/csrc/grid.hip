@@ -151,6 +151,92 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Batched variants: blockIdx.z selects one of `count` transforms of equal
+// mode and fine-grid shape whose grids are contiguous (stride `gstride`).
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kBlock)
+    upsample_batch_kernel(int order_fft, const Cplx<T>* const* __restrict__ ins, Dims3 st,
+                          Dims3 N, FsPtrs<T> fs, Cplx<T>* __restrict__ grids, std::size_t gstride,
+                          Dims3 n) {
+  const Cplx<T>* __restrict__ in = ins[blockIdx.z];
+  Cplx<T>* __restrict__ grid = grids + gstride * blockIdx.z;
+  const IntType rows = (DIM >= 2 ? n.v[1] : 1) * (DIM >= 3 ? n.v[2] : 1);
+  for (IntType row = blockIdx.y; row < rows; row += gridDim.y) {
+    T frow = T(1);
+    bool live = true;
+    IntType off = 0;
+    if constexpr (DIM >= 2) {
+      const IntType g1 = row % n.v[1];
+      IntType k1;
+      live = cell_mode(g1, n.v[1], N.v[1], k1);
+      if (live) {
+        frow *= fs.p[1][k1 < 0 ? -k1 : k1];
+        off += mode_index(k1, N.v[1], order_fft) * st.v[1];
+      }
+      if constexpr (DIM >= 3) {
+        const IntType g2 = row / n.v[1];
+        IntType k2;
+        const bool l2 = cell_mode(g2, n.v[2], N.v[2], k2);
+        live = live && l2;
+        if (live) {
+          frow *= fs.p[2][k2 < 0 ? -k2 : k2];
+          off += mode_index(k2, N.v[2], order_fft) * st.v[2];
+        }
+      }
+    }
+    Cplx<T>* grow = grid + row * n.v[0];
+    for (IntType g0 = threadIdx.x + static_cast<IntType>(blockIdx.x) * kBlock; g0 < n.v[0];
+         g0 += static_cast<IntType>(gridDim.x) * kBlock) {
+      Cplx<T> v{T(0), T(0)};
+      IntType k0;
+      if (live && cell_mode(g0, n.v[0], N.v[0], k0)) {
+        const Cplx<T> a = in[off + mode_index(k0, N.v[0], order_fft) * st.v[0]];
+        const T f = frow * fs.p[0][k0 < 0 ? -k0 : k0];
+        v = Cplx<T>{a.x * f, a.y * f};
+      }
+      grow[g0] = v;
+    }
+  }
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kBlock)
+    downsample_batch_kernel(int order_fft, const Cplx<T>* __restrict__ grids, std::size_t gstride,
+                            Dims3 n, FsPtrs<T> fs, Cplx<T>* const* __restrict__ outs, Dims3 st,
+                            Dims3 N) {
+  const Cplx<T>* __restrict__ grid = grids + gstride * blockIdx.z;
+  Cplx<T>* __restrict__ out = outs[blockIdx.z];
+  const IntType rows = (DIM >= 2 ? N.v[1] : 1) * (DIM >= 3 ? N.v[2] : 1);
+  for (IntType row = blockIdx.y; row < rows; row += gridDim.y) {
+    T frow = T(1);
+    IntType goff = 0, ooff = 0;
+    if constexpr (DIM >= 2) {
+      const IntType i1 = row % N.v[1];
+      const IntType k1 = order_fft ? (i1 < N.v[1] - N.v[1] / 2 ? i1 : i1 - N.v[1]) : i1 - N.v[1] / 2;
+      frow *= fs.p[1][k1 < 0 ? -k1 : k1];
+      goff += (k1 < 0 ? k1 + n.v[1] : k1);
+      ooff += i1 * st.v[1];
+      if constexpr (DIM >= 3) {
+        const IntType i2 = row / N.v[1];
+        const IntType k2 =
+            order_fft ? (i2 < N.v[2] - N.v[2] / 2 ? i2 : i2 - N.v[2]) : i2 - N.v[2] / 2;
+        frow *= fs.p[2][k2 < 0 ? -k2 : k2];
+        goff += (k2 < 0 ? k2 + n.v[2] : k2) * n.v[1];
+        ooff += i2 * st.v[2];
+      }
+    }
+    const Cplx<T>* grow = grid + goff * n.v[0];
+    for (IntType i0 = threadIdx.x + static_cast<IntType>(blockIdx.x) * kBlock; i0 < N.v[0];
+         i0 += static_cast<IntType>(gridDim.x) * kBlock) {
+      const IntType k0 =
+          order_fft ? (i0 < N.v[0] - N.v[0] / 2 ? i0 : i0 - N.v[0]) : i0 - N.v[0] / 2;
+      const Cplx<T> a = grow[k0 < 0 ? k0 + n.v[0] : k0];
+      const T f = frow * fs.p[0][k0 < 0 ? -k0 : k0];
+      out[ooff + i0 * st.v[0]] = Cplx<T>{a.x * f, a.y * f};
+    }
+  }
+}
+
 struct Vec3d {
   double v[3];
 };
@@ -262,6 +348,49 @@ void launch_downsample(int order_fft, const Cplx<T>* grid, const IntType* n, T* 
 }
 
 template <typename T, int DIM>
+void launch_upsample_batch(int order_fft, const Cplx<T>* const* ins, const IntType* in_strides,
+                           const IntType* modes, T* const* fs, Cplx<T>* grids,
+                           std::size_t gstride, const IntType* n, int count, hipStream_t stream) {
+  if (count <= 0) return;
+  Dims3 st{{0, 0, 0}}, NN{{1, 1, 1}}, nn{{1, 1, 1}};
+  FsPtrs<T> f{{nullptr, nullptr, nullptr}};
+  for (int d = 0; d < DIM; ++d) {
+    st.v[d] = in_strides[d];
+    NN.v[d] = modes[d];
+    nn.v[d] = n[d];
+    f.p[d] = fs[d];
+  }
+  const IntType rows = nn.v[1] * nn.v[2];
+  const unsigned gx = static_cast<unsigned>((nn.v[0] + kBlock - 1) / kBlock);
+  const unsigned gy = static_cast<unsigned>(rows < 1024 ? rows : 1024);
+  hipLaunchKernelGGL((upsample_batch_kernel<T, DIM>), dim3(gx, gy, static_cast<unsigned>(count)),
+                     dim3(kBlock), 0, stream, order_fft, ins, st, NN, f, grids, gstride, nn);
+  check_launch("upsample_batch_kernel");
+}
+
+template <typename T, int DIM>
+void launch_downsample_batch(int order_fft, const Cplx<T>* grids, std::size_t gstride,
+                             const IntType* n, T* const* fs, Cplx<T>* const* outs,
+                             const IntType* out_strides, const IntType* modes, int count,
+                             hipStream_t stream) {
+  if (count <= 0) return;
+  Dims3 st{{0, 0, 0}}, NN{{1, 1, 1}}, nn{{1, 1, 1}};
+  FsPtrs<T> f{{nullptr, nullptr, nullptr}};
+  for (int d = 0; d < DIM; ++d) {
+    st.v[d] = out_strides[d];
+    NN.v[d] = modes[d];
+    nn.v[d] = n[d];
+    f.p[d] = fs[d];
+  }
+  const IntType rows = NN.v[1] * NN.v[2];
+  const unsigned gx = static_cast<unsigned>((NN.v[0] + kBlock - 1) / kBlock);
+  const unsigned gy = static_cast<unsigned>(rows < 1024 ? rows : 1024);
+  hipLaunchKernelGGL((downsample_batch_kernel<T, DIM>), dim3(gx, gy, static_cast<unsigned>(count)),
+                     dim3(kBlock), 0, stream, order_fft, grids, gstride, nn, f, outs, st, NN);
+  check_launch("downsample_batch_kernel");
+}
+
+template <typename T, int DIM>
 void launch_prephase(int sign, Loc<T> loc, const uint32_t* perm, std::size_t m,
                      const double* sbar, Cplx<T>* pre_sorted, hipStream_t stream) {
   if (m == 0) return;
@@ -301,7 +430,13 @@ void launch_postphase(int sign, Loc<T> loc, const uint32_t* perm, std::size_t m,
                                         const double*, Cplx<T>*, hipStream_t);                    \
   template void launch_postphase<T, DIM>(int, Loc<T>, const uint32_t*, std::size_t,               \
                                          const double*, const double*, const double*,             \
-                                         const QuadRule&, Cplx<T>*, hipStream_t);
+                                         const QuadRule&, Cplx<T>*, hipStream_t);                 \
+  template void launch_upsample_batch<T, DIM>(int, const Cplx<T>* const*, const IntType*,         \
+                                              const IntType*, T* const*, Cplx<T>*, std::size_t,   \
+                                              const IntType*, int, hipStream_t);                  \
+  template void launch_downsample_batch<T, DIM>(int, const Cplx<T>*, std::size_t, const IntType*, \
+                                                T* const*, Cplx<T>* const*, const IntType*,       \
+                                                const IntType*, int, hipStream_t);
 
 NNF_INST_GRID_T(float)
 NNF_INST_GRID_T(double)

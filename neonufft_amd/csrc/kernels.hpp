@@ -69,6 +69,38 @@ void launch_interp(int ns, const PointRec<T, DIM>* pts, const uint32_t* perm, co
                    std::size_t nsub, const Cplx<T>* grid, const Cplx<T>* post_sorted, Cplx<T>* out,
                    const BinGeom& g, const EsParams& es, hipStream_t stream);
 
+// Per-transform descriptors of the batched kernels (one subproblem table
+// across many independent transforms; Subprob::job indexes these).
+template <typename T, int DIM>
+struct SpreadJob {
+  const PointRec<T, DIM>* pts;
+  const uint32_t* perm;
+  const Cplx<T>* in;
+  const Cplx<T>* pre;  // bin-order prephase or null
+  Cplx<T>* grid;
+  BinGeom g;
+};
+
+template <typename T, int DIM>
+struct InterpJob {
+  const PointRec<T, DIM>* pts;
+  const uint32_t* perm;
+  const Cplx<T>* grid;
+  const Cplx<T>* post;  // bin-order postphase or null
+  Cplx<T>* out;
+  BinGeom g;
+};
+
+template <typename T, int DIM>
+void launch_spread_batch(int ns, const SpreadJob<T, DIM>* jobs, const Subprob* subs,
+                         std::size_t nsub, std::size_t lds_bytes, const EsParams& es,
+                         hipStream_t stream);
+
+template <typename T, int DIM>
+void launch_interp_batch(int ns, const InterpJob<T, DIM>* jobs, const Subprob* subs,
+                         std::size_t nsub, std::size_t lds_bytes, const EsParams& es,
+                         hipStream_t stream);
+
 // Dynamic LDS bytes of one spreading / interpolation block for a bin geometry.
 template <typename T>
 std::size_t tile_bytes(const BinGeom& g);
@@ -91,6 +123,24 @@ template <typename T, int DIM>
 void launch_downsample(int order_fft, const Cplx<T>* grid, const IntType* n, T* const* fs,
                        Cplx<T>* out, const IntType* out_strides, const IntType* modes,
                        hipStream_t stream);
+
+// Batched up/down-sampling over `count` transforms with equal shapes whose fine
+// grids are contiguous with stride `gstride` cells; ins / outs are device
+// arrays of `count` pointers.
+template <typename T, int DIM>
+void launch_upsample_batch(int order_fft, const Cplx<T>* const* ins, const IntType* in_strides,
+                           const IntType* modes, T* const* fs, Cplx<T>* grids,
+                           std::size_t gstride, const IntType* n, int count, hipStream_t stream);
+
+template <typename T, int DIM>
+void launch_downsample_batch(int order_fft, const Cplx<T>* grids, std::size_t gstride,
+                             const IntType* n, T* const* fs, Cplx<T>* const* outs,
+                             const IntType* out_strides, const IntType* modes, int count,
+                             hipStream_t stream);
+
+// Copies a subproblem table into a batched table, tagging every entry with `job`.
+void launch_subprob_retag(const Subprob* src, std::size_t nsub, uint32_t job, Subprob* dst,
+                          hipStream_t stream);
 
 // Type-3 phase factors, written in bin (sorted) order: value[p] belongs to
 // point perm[p].

@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(kBlock)
       sp.bin = static_cast<uint32_t>(b);
       sp.begin = b0 + k * chunk;
       sp.count = min(chunk, c - k * chunk);
-      sp.pad = 0;
+      sp.job = 0;
       subs[s0 + k] = sp;
     }
   }
@@ -267,6 +267,17 @@ __global__ void __launch_bounds__(kOrderThreads)
       pts[sp.begin + pos] = rec[k];
       perm[sp.begin + pos] = idx[k];
     }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    subprob_retag_kernel(const Subprob* __restrict__ src, std::size_t nsub, uint32_t job,
+                         Subprob* __restrict__ dst) {
+  for (std::size_t i = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock; i < nsub;
+       i += static_cast<std::size_t>(gridDim.x) * kBlock) {
+    Subprob s = src[i];
+    s.job = job;
+    dst[i] = s;
   }
 }
 
@@ -358,6 +369,14 @@ void launch_subprob_build(const uint32_t* count, const uint32_t* bin_start,
   hipLaunchKernelGGL(subprob_build_kernel, dim3(grid_for(nbins, kBlock)), dim3(kBlock), 0, stream,
                      count, bin_start, sub_start, nbins, max_points, subs);
   check_launch("subprob_build_kernel");
+}
+
+void launch_subprob_retag(const Subprob* src, std::size_t nsub, uint32_t job, Subprob* dst,
+                          hipStream_t stream) {
+  if (nsub == 0) return;
+  hipLaunchKernelGGL(subprob_retag_kernel, dim3(grid_for(nsub, kBlock)), dim3(kBlock), 0, stream,
+                     src, nsub, job, dst);
+  check_launch("subprob_retag_kernel");
 }
 
 int minmax_blocks(std::size_t m) {

@@ -1,0 +1,178 @@
+"""GPU tests of the batched plan (many uneven type 1 / type 2 transforms in one
+set of launches), through the C ABI.
+
+The batched plan has no reference counterpart; its contract is that each
+transform gives what a separate plan of the reference's interface
+(include/neonufft/gpu/plan.hpp:25-46) gives for the same points, modes and
+options. So every transform is compared with a plain ``Plan`` run alone
+(same kernels, so agreement is to rounding of the float atomics' order), and
+one transform per case with the FP64 direct sum (closed-form truth).
+Graph replays are checked against eager runs with changing buffers and
+unrelated eager work between the replays.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import neonufft_amd as nnf
+from oracle import oracle as O
+
+from test_gpu_plan import CDT, RDT, bound, cplx, dev, points
+
+pytestmark = pytest.mark.gpu
+
+# per transform: (modes, points); repeated shapes share one batched FFT
+SHAPES = {
+    1: [((64,), 300), ((17,), 0), ((64,), 1000), ((200,), 57), ((5,), 40)],
+    2: [((32, 24), 300), ((8, 8), 0), ((32, 24), 1500), ((60, 2), 77), ((8, 8), 900),
+        ((1, 40), 33)],
+    3: [((12, 10, 8), 400), ((16, 16, 16), 2500), ((12, 10, 8), 0), ((4, 20, 6), 300)],
+}
+
+# relative l2 distance allowed between a batched transform and a plain plan
+SAME = {"f32": 2e-6, "f64": 1e-13}
+
+
+def _rel(a, b):
+    a = a.ravel()
+    b = b.ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def _setup(dim, prec, seed):
+    rng = np.random.default_rng(seed)
+    spec = SHAPES[dim]
+    locs, cs, fs = [], [], []
+    for modes, m in spec:
+        x = points(rng, dim, m, prec)
+        locs.append([dev(v, RDT[prec]) for v in x])
+        cs.append(dev(cplx(rng, m, prec), CDT[prec]))
+        nm = int(np.prod(modes))
+        fs.append(dev(cplx(rng, nm, prec), CDT[prec]))
+    return spec, locs, cs, fs
+
+
+def _to_np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("order", [nnf.ModeOrder.CMCL, nnf.ModeOrder.FFT])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_batch_matches_plans(dim, prec, order, sign):
+    tol = 1e-5 if prec == "f32" else 1e-9
+    opt = nnf.Options(tol=tol, order=order)
+    spec, locs, cs, fs = _setup(dim, prec, 10 * dim + (sign > 0))
+    num = [m for _, m in spec]
+    modes = [list(md) for md, _ in spec]
+    batch = nnf.BatchPlan(opt, sign, num, locs, modes, dtype=RDT[prec], use_graph=False)
+    info = batch.info()
+    assert info["count"] == len(spec)
+    assert info["num_points"] == sum(num)
+    assert info["num_shapes"] == len({tuple(md) for md, _ in spec})
+
+    outs1 = [torch.empty(int(np.prod(md)), dtype=CDT[prec], device="cuda") for md, _ in spec]
+    outs2 = [torch.empty(max(m, 1), dtype=CDT[prec], device="cuda") for _, m in spec]
+    batch.transform_type_1(cs, outs1)
+    batch.transform_type_2(fs, outs2)
+    torch.cuda.synchronize()
+
+    for t, (md, m) in enumerate(spec):
+        plan = nnf.Plan(opt, sign, m, locs[t], md, dtype=RDT[prec])
+        r1 = torch.empty_like(outs1[t])
+        plan.transform_type_1(cs[t], r1)
+        r2 = torch.empty_like(outs2[t])
+        plan.transform_type_2(fs[t], None, r2)
+        torch.cuda.synchronize()
+        if m == 0:
+            assert np.all(_to_np(outs1[t]) == 0), f"transform {t}: no points must give zeros"
+        else:
+            assert _rel(_to_np(outs1[t]), _to_np(r1)) < SAME[prec], f"type 1 transform {t}"
+            assert _rel(_to_np(outs2[t])[:m], _to_np(r2)[:m]) < SAME[prec], f"type 2 transform {t}"
+
+    # one transform against the FP64 direct sum
+    t = max(range(len(spec)), key=lambda k: spec[k][1])
+    md, m = spec[t]
+    xs = [_to_np(v).astype(np.float64) for v in locs[t]]
+    kk = O.mode_vectors(md, np.arange(int(np.prod(md))),
+                        order_fft=nnf.ModeOrder(order) == nnf.ModeOrder.FFT)
+    truth1 = O.direct_type1(xs, _to_np(cs[t]).astype(np.complex128), kk, sign)
+    err = _rel(_to_np(outs1[t]).astype(np.complex128), truth1)
+    assert err < bound(prec, tol, 1) * 20, f"type 1 vs direct: {err}"
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_batch_graph_replays(prec):
+    """Captured graphs replay correctly when the caller's buffers change
+    between calls, with unrelated eager work in between."""
+    dim = 2
+    opt = nnf.Options(tol=1e-5 if prec == "f32" else 1e-9)
+    spec, locs, cs, fs = _setup(dim, prec, 99)
+    num = [m for _, m in spec]
+    modes = [list(md) for md, _ in spec]
+    gb = nnf.BatchPlan(opt, -1, num, locs, modes, dtype=RDT[prec], use_graph=True)
+    eb = nnf.BatchPlan(opt, -1, num, locs, modes, dtype=RDT[prec], use_graph=False)
+    rng = np.random.default_rng(5)
+    for rep in range(4):
+        cin = [dev(cplx(rng, max(m, 0), prec), CDT[prec]) for _, m in spec]
+        fin = [dev(cplx(rng, int(np.prod(md)), prec), CDT[prec]) for md, _ in spec]
+        g1 = [torch.empty(int(np.prod(md)), dtype=CDT[prec], device="cuda") for md, _ in spec]
+        e1 = [torch.empty_like(v) for v in g1]
+        g2 = [torch.empty(max(m, 1), dtype=CDT[prec], device="cuda") for _, m in spec]
+        e2 = [torch.empty_like(v) for v in g2]
+        gb.transform_type_1(cin, g1)
+        # unrelated eager work on the same stream between the replays
+        junk = torch.randn(1 << 16, device="cuda").cumsum(0)
+        gb.transform_type_2(fin, g2)
+        eb.transform_type_1(cin, e1)
+        eb.transform_type_2(fin, e2)
+        torch.cuda.synchronize()
+        assert float(junk[-1].abs()) >= 0.0
+        for t, (md, m) in enumerate(spec):
+            assert _rel(_to_np(g1[t]), _to_np(e1[t])) < SAME[prec], f"replay {rep} type 1 t={t}"
+            if m:
+                assert _rel(_to_np(g2[t])[:m], _to_np(e2[t])[:m]) < SAME[prec], \
+                    f"replay {rep} type 2 t={t}"
+        assert gb.info()["graph"], "graph capture did not happen"
+        assert not eb.info()["graph"]
+
+
+def test_batch_set_points():
+    """New points (and point counts) through set_points match a fresh plan."""
+    prec, dim = "f32", 3
+    opt = nnf.Options(tol=1e-5)
+    spec, locs, cs, fs = _setup(dim, prec, 3)
+    modes = [list(md) for md, _ in spec]
+    batch = nnf.BatchPlan(opt, 1, [m for _, m in spec], locs, modes, dtype=RDT[prec])
+    rng = np.random.default_rng(8)
+    num2 = [120, 0, 3000, 45]
+    locs2 = [[dev(v, RDT[prec]) for v in points(rng, dim, m, prec)] for m in num2]
+    cs2 = [dev(cplx(rng, m, prec), CDT[prec]) for m in num2]
+    batch.set_points(num2, locs2)
+    outs = [torch.empty(int(np.prod(md)), dtype=CDT[prec], device="cuda") for md, _ in spec]
+    batch.transform_type_1(cs2, outs)
+    for t, (md, _) in enumerate(spec):
+        plan = nnf.Plan(opt, 1, num2[t], locs2[t], md, dtype=RDT[prec])
+        r = torch.empty_like(outs[t])
+        plan.transform_type_1(cs2[t], r)
+        torch.cuda.synchronize()
+        if num2[t]:
+            assert _rel(_to_np(outs[t]), _to_np(r)) < SAME[prec]
+        else:
+            assert np.all(_to_np(outs[t]) == 0)
+    assert batch.info()["num_points"] == sum(num2)
+
+
+def test_batch_input_errors():
+    opt = nnf.Options(tol=1e-5)
+    x = torch.zeros(4, device="cuda")
+    with pytest.raises(nnf.InputError):
+        nnf.BatchPlan(opt, -1, [4], [[x, x]], [[8, 0]])
+    with pytest.raises(nnf.InputError):
+        nnf.BatchPlan(opt, -1, [4, 4], [[x]], [[8]])
+    b = nnf.BatchPlan(opt, -1, [4], [[x]], [[8]])
+    with pytest.raises(nnf.InputError):
+        b.transform_type_1([x], [])
