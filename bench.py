@@ -10,8 +10,12 @@ inputs are resident in HBM. value = NU points processed per second
 
 Prints ONE JSON line. Extra keys: the spreading / interpolation kernel
 rooflines (HIP events on the plan's stream), a bounded CPU baseline (the FP64
-C restatement in oracle/, OpenMP), and the relative l2 error of both
-transforms on 1000 sampled outputs against the FP64 direct NUDFT.
+C restatement in oracle/, OpenMP), the relative l2 error of both
+transforms on 1000 sampled outputs against the FP64 direct NUDFT, and the
+batched-uneven comparison: 256 independent 2D type-1 transforms (complex64,
+point counts log-uniform in [1e4, 1e6], modes drawn from {256^2, 512^2,
+1024^2}, tol 1e-5) through one BatchPlan (one spreading launch, batched FFTs,
+HIP graph) against 256 sequential plain plans on the same GPU.
 """
 from __future__ import annotations
 
@@ -32,6 +36,13 @@ SEED = 0x4E554654
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--workload", default="uniform",
+                   choices=["uniform", "clustered", "type3", "batched"],
+                   help="uniform: 3D type 1+2, 1e7 points, 128^3 modes (default, plus the "
+                        "batched comparison); clustered: 3D type 2, 2e7 points in 16 "
+                        "Gaussian clusters, 256^3 modes, tol 1e-4 and 1e-6; type3: 3D type 3, "
+                        "2e6 sources, 1e6 targets, tol 1e-6; batched: the batched comparison "
+                        "alone")
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--points", type=int, default=10_000_000)
@@ -41,6 +52,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=4_000_000,
                    help="points in the bounded CPU-baseline sample")
+    p.add_argument("--batch-count", type=int, default=256,
+                   help="transforms in the batched-uneven comparison (0 = skip it)")
+    p.add_argument("--batch-reps", type=int, default=5)
     p.add_argument("--calib-copy-mib", type=int, default=0,
                    help="also run a plain device copy of this many MiB (profiler calibration)")
     return p.parse_args()
@@ -85,13 +99,387 @@ def cpu_baseline_leg(x, c, f_gpu, out2_gpu, modes, tol, sample):
     return cpu, err1, err2
 
 
+def batched_leg(args, dev, stream):
+    """Batched uneven transforms vs the same transforms as sequential plain
+    plans. Every timed call has its points set and inputs resident in HBM."""
+    import numpy as np
+    import torch
+
+    import neonufft_amd as nnf
+
+    count = args.batch_count
+    rng = np.random.default_rng(SEED + 5)
+    ms = np.floor(10.0 ** rng.uniform(4.0, 6.0, count)).astype(np.int64)
+    nmodes = rng.choice([256, 512, 1024], size=count)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(SEED + 5)
+    locs, cs, outs_b, outs_p = [], [], [], []
+    for t in range(count):
+        m = int(ms[t])
+        xy = torch.rand(2, m, generator=gen, device=dev) * (2 * math.pi) - math.pi
+        locs.append([xy[0], xy[1]])
+        cs.append(torch.view_as_complex(torch.randn(m, 2, generator=gen, device=dev)))
+        outs_b.append(torch.empty(int(nmodes[t]) ** 2, dtype=torch.complex64, device=dev))
+        outs_p.append(torch.empty_like(outs_b[-1]))
+    opt = nnf.Options(tol=1e-5, upsampfac=2.0)
+    modes = [[int(n), int(n)] for n in nmodes]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch = nnf.BatchPlan(opt, -1, [int(v) for v in ms], locs, modes, stream=stream,
+                          use_graph=True)
+    torch.cuda.synchronize()
+    batch_create_ms = (time.perf_counter() - t0) * 1e3
+    plans = [nnf.Plan(opt, -1, int(ms[t]), locs[t], modes[t], stream=stream)
+             for t in range(count)]
+    torch.cuda.synchronize()
+
+    def run_batched():
+        batch.transform_type_1(cs, outs_b)
+
+    def run_plain():
+        for t in range(count):
+            plans[t].transform_type_1(cs[t], outs_p[t])
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.batch_reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / args.batch_reps
+
+    batched_ms = timed(run_batched)
+    plain_ms = timed(run_plain)
+    torch.cuda.synchronize()
+    info = batch.info()
+
+    # every batched transform against its plain plan (same kernels)
+    max_diff = 0.0
+    for t in range(count):
+        a, b = outs_b[t], outs_p[t]
+        max_diff = max(max_diff, float(torch.linalg.vector_norm(a - b)
+                                       / torch.linalg.vector_norm(b)))
+    # FP64 direct sum on 1000 sampled modes of the transform with fewest points
+    t = int(np.argmin(ms))
+    n = int(nmodes[t])
+    x = locs[t][0].double().cpu().numpy()
+    y = locs[t][1].double().cpu().numpy()
+    c = cs[t].cpu().numpy().astype(np.complex128)
+    idx = np.sort(rng.choice(n * n, size=1000, replace=False))
+    kx = (idx % n - n // 2).astype(np.float64)
+    ky = (idx // n - n // 2).astype(np.float64)
+    ref = np.zeros(len(idx), dtype=np.complex128)
+    for j0 in range(0, len(x), 4096):
+        sl = slice(j0, j0 + 4096)
+        ref += np.exp(-1j * (np.outer(kx, x[sl]) + np.outer(ky, y[sl]))) @ c[sl]
+    got = outs_b[t].cpu().numpy()[idx].astype(np.complex128)
+    direct_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+    total = int(ms.sum())
+    out = {
+        "workload": (f"{count} independent 2D type-1 transforms, complex64, point counts "
+                     f"floor(10^U[4,6]) (sum {total}), modes from {{256^2, 512^2, 1024^2}}, "
+                     f"tol 1e-5, upsampfac 2.0"),
+        "transforms": count,
+        "sum_points": total,
+        "shapes": info["num_shapes"],
+        "grid_cells": info["grid_cells"],
+        "subproblems": info["num_subproblems"],
+        "graph": info["graph"],
+        "batched_ms": batched_ms,
+        "plain_ms": plain_ms,
+        "speedup": plain_ms / batched_ms,
+        "batched_pts_per_s": total / (batched_ms * 1e-3),
+        "plain_pts_per_s": total / (plain_ms * 1e-3),
+        "batch_create_ms": batch_create_ms,
+        "max_rel_diff_vs_plain": max_diff,
+        "direct_rel_l2": direct_err,
+        "direct_check": f"1000 sampled modes of transform {t} (M={int(ms[t])}, {n}^2 modes)",
+    }
+    del plans, batch
+    torch.cuda.synchronize()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: --gpus must be 1 "
                           "(multi-GPU sharding is a design note, not a measured path)"}))
         sys.exit(2)
+    if args.workload == "clustered":
+        return run_clustered(args)
+    if args.workload == "type3":
+        return run_type3(args)
+    if args.workload == "batched":
+        return run_batched(args)
+    return run_uniform(args)
 
+
+def _timed(stream, fn, reps):
+    import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    fn()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _common(args, metric, value, ms_step, dtype, config):
+    return {
+        "metric": metric, "value": value, "unit": "NU pts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic (seeded)", "config": config,
+    }
+
+
+def _roofline(kernel, alg_bytes, ms, traffic=None):
+    gbs = alg_bytes / (ms * 1e-3)
+    return {"bound": "hbm", "kernel": kernel, "achieved": gbs / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": gbs / HBM_PEAK, "traffic": traffic,
+            "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": ms}
+
+
+def clustered_points(rng, m):
+    """Mixture of 16 isotropic Gaussians (sigma 0.05, centres uniform in
+    [-pi, pi)^3), folded periodically into [-pi, pi)."""
+    import numpy as np
+    centres = rng.uniform(-math.pi, math.pi, (16, 3))
+    which = rng.integers(0, 16, m)
+    out = []
+    for d in range(3):
+        v = centres[which, d] + 0.05 * rng.standard_normal(m)
+        v = np.mod(v + math.pi, 2 * math.pi) - math.pi
+        out.append(v.astype(np.float32))
+    return out
+
+
+def cpu_leg_type2(x, f, out_gpu, modes, tol, sample, nacc):
+    """Checker work on the host for a type-2 workload: FP64 direct-sum error
+    on `nacc` sampled points and the timed CPU restatement (oracle/) on the
+    first `sample` points."""
+    import numpy as np
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(SEED + 98)
+    m = x[0].shape[0]
+    xs = [v.astype(np.float64) for v in x]
+    f64 = f.astype(np.complex128)
+    pidx = np.sort(rng.choice(m, size=min(nacc, m), replace=False))
+    ref = O.direct_type2([v[pidx] for v in xs], f64, modes, -1)
+    err = O.rel_l2(out_gpu[pidx], ref)
+    ms = min(sample, m)
+    t0 = time.perf_counter()
+    O.nufft2([v[:ms] for v in xs], f64, modes, -1, tol)
+    t1 = time.perf_counter()
+    cpu = {"value": ms / (t1 - t0), "unit": "NU pts/s", "cores": O.num_threads(),
+           "kind": "port",
+           "sample": (f"FP64 C restatement (oracle/, OpenMP) + numpy FFT, type 2 on the first "
+                      f"{ms} points of the workload, same grid: {t1 - t0:.2f}s")}
+    return cpu, err
+
+
+def run_clustered(args):
+    import numpy as np
+    import torch
+
+    import neonufft_amd as nnf
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    m = 20_000_000
+    N = 256
+    modes = (N, N, N)
+    rng = np.random.default_rng(SEED + 3)
+    x = clustered_points(rng, m)
+    f = (rng.standard_normal(N ** 3, dtype=np.float32)
+         + 1j * rng.standard_normal(N ** 3, dtype=np.float32)).astype(np.complex64)
+    dx = [torch.from_numpy(v).to(dev) for v in x]
+    df = torch.from_numpy(f).to(dev)
+    out = torch.empty(m, dtype=torch.complex64, device=dev)
+    results = []
+    for tol in (1e-4, 1e-6):
+        opt = nnf.Options(tol=tol, upsampfac=2.0)
+        t0 = time.perf_counter()
+        tune = nnf.autotune(opt, -1, m, dx, modes, types=2, reps=3, stream=stream)
+        tune_s = time.perf_counter() - t0
+        plan = nnf.Plan(opt, -1, m, dx, modes, stream=stream, bin_size=tune["bin_size"],
+                        max_points=tune["max_points"])
+        base = nnf.Plan(opt, -1, m, dx, modes, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.set_points(m, dx)
+        torch.cuda.synchronize()
+        set_ms = (time.perf_counter() - t0) * 1e3
+        step = lambda: plan.transform_type_2(df, None, out)  # noqa: E731
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        ms_step = (time.perf_counter() - t0) / args.steps * 1e3
+        interp_ms = _timed(stream, lambda: plan.interp(out), args.kernel_reps)
+        base_interp_ms = _timed(stream, lambda: base.interp(out), args.kernel_reps)
+        plan.transform_type_2(df, None, out)
+        torch.cuda.synchronize()
+        info = plan.info()
+        G = int(np.prod(info["grid"]))
+        results.append({
+            "tol": tol, "n_spread": info["n_spread"], "ms_per_step": ms_step,
+            "pts_per_s": m / (ms_step * 1e-3), "set_points_ms": set_ms,
+            "interp_ms": interp_ms, "default_shape_interp_ms": base_interp_ms,
+            "tuned": {"bin_size": tune["bin_size"], "max_points": tune["max_points"],
+                      "tuning_s": tune_s},
+            "subproblems": info["num_subproblems"],
+            "roofline": _roofline("interp_kernel", m * (3 * 4 + 8) + G * 8, interp_ms),
+            "out": out.cpu().numpy() if not args.no_cpu_baseline else None,
+        })
+        del plan, base
+    cpu = None
+    for r in results:
+        o = r.pop("out")
+        if o is not None:
+            c, err = cpu_leg_type2(x, f, o, modes, r["tol"], args.cpu_sample // 2, 200)
+            r["rel_l2_vs_direct"] = err
+            r["accuracy_samples"] = 200
+            if cpu is None:
+                cpu = c
+    head = results[0]
+    line = _common(args, "NU pts/s, 3D type-2 fp32, 2e7 clustered points, 256^3 modes, 1 GPU",
+                   head["pts_per_s"], head["ms_per_step"], "f32",
+                   {"workload": "3D type-2, complex64, M=2e7 points in 16 Gaussian clusters "
+                                "(sigma 0.05, folded), modes 256^3 (fine grid 512^3), "
+                                "tol 1e-4 and 1e-6, autotuned tiles",
+                    "points": m, "modes": list(modes)})
+    line["roofline"] = head["roofline"]
+    line["sweep"] = results
+    line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+def run_type3(args):
+    import numpy as np
+    import torch
+
+    import neonufft_amd as nnf
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    m, n = 2_000_000, 1_000_000
+    rng = np.random.default_rng(SEED + 4)
+    x = [rng.uniform(-1e4, 1e4, m), rng.uniform(-1e4, 1e4, m), rng.uniform(-50, 50, m)]
+    s = [rng.uniform(-0.02, 0.02, n), rng.uniform(-0.02, 0.02, n), rng.uniform(-1, 1, n)]
+    c = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    res = {}
+    outs = {}
+    for prec, rdt, cdt in (("mixed", torch.float64, torch.complex128),
+                           ("f32", torch.float32, torch.complex64),
+                           ("f64", torch.float64, torch.complex128)):
+        dx = [torch.from_numpy(v).to(dev, rdt) for v in x]
+        ds = [torch.from_numpy(v).to(dev, rdt) for v in s]
+        dc = torch.from_numpy(c).to(dev, cdt)
+        out = torch.empty(n, dtype=cdt, device=dev)
+        opt = nnf.Options(tol=1e-6, upsampfac=2.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan = nnf.PlanT3(opt, -1, m, dx, n, ds, stream=stream,
+                          dtype="mixed" if prec == "mixed" else None)
+        torch.cuda.synchronize()
+        create_ms = (time.perf_counter() - t0) * 1e3
+
+        def step():
+            plan.reset()
+            plan.add_input(dc)
+            plan.transform(out)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        ms_step = (time.perf_counter() - t0) / args.steps * 1e3
+        add_ms = _timed(stream, lambda: plan.add_input(dc), args.kernel_reps)
+        tr_ms = _timed(stream, lambda: plan.transform(out), args.kernel_reps)
+        step()
+        torch.cuda.synchronize()
+        info = plan.info()
+        Gs = int(np.prod(info["spread_grid"]))
+        cb = 4 if prec == "f32" else 8  # bytes per coordinate and per input complex part
+        gb = 16 if prec == "f64" else 8  # bytes per spread-grid cell
+        res[prec] = {
+            "ms_per_step": ms_step, "pts_per_s": (m + n) / (ms_step * 1e-3),
+            "plan_create_ms": create_ms, "add_input_ms": add_ms, "transform_ms": tr_ms,
+            "n_spread": info["n_spread"], "spread_grid": info["spread_grid"],
+            "fft_grid": info["grid"],
+            "roofline": _roofline("spread_kernel (add_input)",
+                                  m * (3 * cb + 2 * cb) + Gs * gb, add_ms),
+        }
+        outs[prec] = out.cpu().numpy()
+        del plan
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        rng2 = np.random.default_rng(SEED + 97)
+        tidx = np.sort(rng2.choice(n, size=1000, replace=False))
+        for prec, dt in (("mixed", np.float64), ("f32", np.float32), ("f64", np.float64)):
+            xs = [v.astype(dt).astype(np.float64) for v in x]
+            ss = [v[tidx].astype(dt).astype(np.float64) for v in s]
+            cc = c.astype(np.complex64 if prec == "f32" else np.complex128).astype(np.complex128)
+            ref = O.direct_type3(xs, cc, ss, -1)
+            res[prec]["rel_l2_vs_direct"] = O.rel_l2(outs[prec][tidx], ref)
+        ms_, ns_ = args.cpu_sample // 20, args.cpu_sample // 40
+        t0 = time.perf_counter()
+        O.nufft3([v[:ms_] for v in x], c[:ms_], [v[:ns_] for v in s], -1, 1e-6)
+        t1 = time.perf_counter()
+        cpu = {"value": (ms_ + ns_) / (t1 - t0), "unit": "NU pts/s", "cores": O.num_threads(),
+               "kind": "port",
+               "sample": (f"FP64 C restatement (oracle/, OpenMP) + numpy FFT, type 3 with the "
+                          f"first {ms_} sources and {ns_} targets: {t1 - t0:.2f}s")}
+    line = _common(args, "NU pts/s (sources + targets), 3D type-3 @1e-6, 1 GPU",
+                   res["mixed"]["pts_per_s"], res["mixed"]["ms_per_step"], "f32",
+                   {"workload": "3D type-3, M=2e6 sources x,y in U[-1e4,1e4], z in U[-50,50]; "
+                                "N=1e6 targets s_x,s_y in U[-0.02,0.02], s_z in U[-1,1]; "
+                                "tol 1e-6; mixed plan (double coordinates, complex128 I/O, "
+                                "FP64 fold and phases, fp32 spreading / FFT / interpolation); "
+                                "fp32 and fp64 plans alongside",
+                    "sources": m, "targets": n})
+    line["roofline"] = res["mixed"]["roofline"]
+    line["precisions"] = res
+    line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+def run_batched(args):
+    import torch
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    b = batched_leg(args, dev, torch.cuda.current_stream())
+    line = _common(args, "NU pts/s, 256 batched uneven 2D type-1 fp32 @1e-5 vs plain, 1 GPU",
+                   b["batched_pts_per_s"], b["batched_ms"], "f32",
+                   {"workload": b["workload"]})
+    line["batched"] = b
+    line["roofline"] = None
+    line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+def run_uniform(args):
     import numpy as np
     import torch
 
@@ -189,6 +577,12 @@ def main():
         cpu, err1, err2 = cpu_baseline_leg(x, c, f_host, out2_host, modes, args.tol,
                                            args.cpu_sample)
 
+    batched = None
+    if args.batch_count > 0:
+        del plan
+        torch.cuda.synchronize()
+        batched = batched_leg(args, dev, stream)
+
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
@@ -240,6 +634,7 @@ def main():
         },
         "accuracy": {"type1_rel_l2": err1, "type2_rel_l2": err2,
                      "samples": 1000, "reference": "FP64 direct NUDFT"},
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

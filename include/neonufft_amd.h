@@ -9,7 +9,10 @@
  *
  * Conventions:
  *  - precision: NNF_F32 (T = float, complex = 2 x float interleaved) or
- *    NNF_F64 (T = double). dim: 1, 2 or 3.
+ *    NNF_F64 (T = double). dim: 1, 2 or 3. Type-3 plans also take NNF_MIXED
+ *    (extension, no reference counterpart): double coordinates and complex128
+ *    inputs / outputs, FP64 fold and phase arguments, fp32 spreading, FFT and
+ *    interpolation.
  *  - Every data pointer is a DEVICE pointer. Coordinate arrays are passed as
  *    an array of `dim` pointers to contiguous T arrays (struct of arrays),
  *    as in the reference's std::array<const T*, DIM>.
@@ -33,6 +36,7 @@ extern "C" {
 
 #define NNF_F32 0
 #define NNF_F64 1
+#define NNF_MIXED 2
 
 /* Mirrors neonufft::Options (include/neonufft/plan.hpp:19-29 of the reference),
  * with the C++ bools as ints. order: 0 = NEONUFFT_MODE_ORDER_FFT,
@@ -139,6 +143,20 @@ typedef struct nnf_tuning {
 int nnf_plan_create_tuned(nnf_plan* plan, int precision, int dim, const nnf_options* opt,
                           int sign, int64_t num_nu, const void* const* loc, const int64_t* modes,
                           void* stream, const nnf_tuning* tuning);
+
+/* ---- Autotuner (no reference counterpart) ---------------------------------
+ * Times the spreading (types bit 0) and / or interpolation (bit 1) kernels of
+ * a plan with these arguments for a list of subgrid tile shapes and
+ * per-workgroup point caps, on the given device points, and returns the
+ * fastest in `best` (pass it to nnf_plan_create_tuned) with its time per
+ * call in `best_ms` (sum over the selected kernels). `reps` timed calls per
+ * candidate. With use_cache non-zero, a result is reused for the same
+ * precision, dim, kernel width, fine grid, floor(log2 num_nu) and types in
+ * this process. */
+int nnf_autotune(int precision, int dim, const nnf_options* opt, int sign, int64_t num_nu,
+                 const void* const* loc, const int64_t* modes, void* stream, int types, int reps,
+                 int use_cache, nnf_tuning* best, double* best_ms);
+int nnf_autotune_clear_cache(void);
 
 /* ---- Stage-level extension (no reference counterpart) --------------------
  * The reference runs these stages only inside transform_type_{1,2}

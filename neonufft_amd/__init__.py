@@ -18,7 +18,7 @@ from typing import Optional, Sequence, Union
 from . import _lib as _L
 
 __all__ = [
-    "Options", "ModeOrder", "Plan", "PlanT3", "BatchPlan", "GenericError", "InputError", "InternalError",
+    "Options", "ModeOrder", "Plan", "PlanT3", "BatchPlan", "autotune", "GenericError", "InputError", "InternalError",
     "MemoryAllocError", "NotImplementedError_", "GPUError", "UnknownError", "version",
 ]
 
@@ -147,7 +147,30 @@ def _dtype_code(dtype) -> int:
         return 0
     if dtype in ("float64", "f64", torch.float64, torch.complex128):
         return 1
+    if dtype == "mixed":
+        # type 3 only: double coordinates, complex128 I/O, fp32 bulk work
+        return 2
     raise InputError(f"unsupported dtype {dtype}")
+
+
+def autotune(opt: "Options", sign: int, num_nu: int, loc: Sequence[Pointer],
+             modes: Sequence[int], types: int = 3, reps: int = 3, stream=None, dtype=None,
+             use_cache: bool = True) -> dict:
+    """Fastest subgrid tile and per-workgroup point cap for a plan with these
+    arguments, timed on these device points (extension; no reference
+    counterpart). types: 1 = spreading, 2 = interpolation, 3 = both.
+    Returns {"bin_size": [...], "max_points": int, "ms": float}."""
+    L = _L.lib()
+    dim = len(modes)
+    prec = _dtype_code(dtype) if dtype is not None else _precision_of(loc[0])
+    vpp, _keep = _vpp([_ptr(p) for p in loc])
+    best = _L.NnfTuning()
+    ms = ctypes.c_double()
+    _check(L.nnf_autotune(prec, dim, ctypes.byref(opt._c()), int(sign), int(num_nu), vpp,
+                          _i64p(modes), ctypes.c_void_p(_stream(stream)), int(types), int(reps),
+                          int(bool(use_cache)), ctypes.byref(best), ctypes.byref(ms)))
+    return {"bin_size": list(best.bin_size)[:dim], "max_points": int(best.max_points),
+            "ms": float(ms.value)}
 
 
 class Plan:
@@ -161,8 +184,14 @@ class Plan:
     def __init__(self, opt: Options, sign: int, num_nu: int, loc: Sequence[Pointer],
                  modes: Sequence[int], stream=None, dtype=None, bin_size=None,
                  max_points: int = 0):
-        """bin_size / max_points: optional tuning (extension; None/0 = defaults)."""
+        """bin_size / max_points: optional tuning (extension; None/0 = defaults).
+        bin_size="auto" runs autotune() (both kernels) on these points first."""
         self._L = _L.lib()
+        if isinstance(bin_size, str):
+            if bin_size != "auto":
+                raise InputError(f"bin_size must be a sequence or 'auto', not {bin_size!r}")
+            t = autotune(opt, sign, num_nu, loc, modes, stream=stream, dtype=dtype)
+            bin_size, max_points = t["bin_size"], t["max_points"]
         self.dim = len(modes)
         self.modes = [int(m) for m in modes]
         self.precision = _dtype_code(dtype) if dtype is not None else _precision_of(loc[0])
@@ -407,7 +436,7 @@ class BatchPlan:
 
 def _info_dict(i: _L.NnfPlanInfo) -> dict:
     return {
-        "precision": "float32" if i.precision == 0 else "float64",
+        "precision": {0: "float32", 1: "float64", 2: "mixed"}.get(i.precision, "?"),
         "dim": i.dim,
         "n_spread": i.n_spread,
         "es_beta": i.es_beta,

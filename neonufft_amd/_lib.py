@@ -19,6 +19,7 @@ EXPORTED = (
     "nnf_plan_fine_grid", "nnf_plan_fseries", "nnf_plan_create_tuned",
     "nnf_batch_create", "nnf_batch_set_points", "nnf_batch_transform_type_1",
     "nnf_batch_transform_type_2", "nnf_batch_get_info", "nnf_batch_destroy",
+    "nnf_autotune", "nnf_autotune_clear_cache",
 )
 
 
@@ -133,6 +134,9 @@ def lib():
     L.nnf_batch_transform_type_2.argtypes = [vp, vpp, vpp]
     L.nnf_batch_get_info.argtypes = [vp, ctypes.POINTER(NnfBatchInfo)]
     L.nnf_batch_destroy.argtypes = [vp]
+    L.nnf_autotune.argtypes = [c_int, c_int, optp, c_int, i64, vpp, i64p, vp, c_int, c_int, c_int,
+                               ctypes.POINTER(NnfTuning), dp]
+    L.nnf_autotune_clear_cache.argtypes = []
     for name in EXPORTED:
         if name not in ("nnf_last_error", "nnf_version", "nnf_options_default"):
             getattr(L, name).restype = ctypes.c_int

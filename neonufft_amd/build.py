@@ -42,6 +42,7 @@ SOURCES = [
     ("params.cpp", False),
     ("fft.cpp", False),
     ("plan.cpp", False),
+    ("autotune.cpp", False),
     ("c_api.cpp", False),
 ]
 

@@ -9,6 +9,13 @@
 
 #include <neonufft_amd.h>
 #include "batch.hpp"
+
+namespace nnf {
+void autotune(int precision, int dim, const neonufft::Options& opt, int sign, IntType num_nu,
+              const void* const* loc, const IntType* modes, hipStream_t stream, int types,
+              int reps, bool use_cache, nnf_tuning* best, double* best_ms);
+void autotune_clear_cache();
+}  // namespace nnf
 #include "plan_impl.hpp"
 
 namespace {
@@ -65,29 +72,30 @@ struct PlanT3Iface {
   virtual void info(nnf_plan_info* i) = 0;
 };
 
-template <typename T, int DIM>
+// TC: coordinate and I/O precision (differs from T for NNF_MIXED).
+template <typename T, int DIM, typename TC = T>
 struct PlanT3Holder final : PlanT3Iface {
-  nnf::PlanT3Impl<T, DIM> impl;
+  nnf::PlanT3Impl<T, DIM, TC> impl;
   PlanT3Holder(const neonufft::Options& o, int sign, IntType num_in, const void* const* in_pts,
                IntType num_out, const void* const* out_pts, hipStream_t s, IntType batch,
                std::shared_ptr<neonufft::Allocator> a)
-      : impl(o, sign, num_in, reinterpret_cast<const T* const*>(in_pts), num_out,
-             reinterpret_cast<const T* const*>(out_pts), s, batch, std::move(a)) {}
+      : impl(o, sign, num_in, reinterpret_cast<const TC* const*>(in_pts), num_out,
+             reinterpret_cast<const TC* const*>(out_pts), s, batch, std::move(a)) {}
   PlanT3Holder(const neonufft::Options& o, int sign, const double* in_min, const double* in_max,
                const double* out_min, const double* out_max, hipStream_t s, IntType batch,
                std::shared_ptr<neonufft::Allocator> a)
       : impl(o, sign, in_min, in_max, out_min, out_max, s, batch, std::move(a)) {}
   void set_in(int64_t n, const void* const* p) override {
-    impl.set_input_points(n, reinterpret_cast<const T* const*>(p));
+    impl.set_input_points(n, reinterpret_cast<const TC* const*>(p));
   }
   void set_out(int64_t n, const void* const* p) override {
-    impl.set_output_points(n, reinterpret_cast<const T* const*>(p));
+    impl.set_output_points(n, reinterpret_cast<const TC* const*>(p));
   }
   void add_input(const void* in, int64_t bdist) override {
-    impl.add_input(static_cast<const Cplx<T>*>(in), bdist);
+    impl.add_input(static_cast<const Cplx<TC>*>(in), bdist);
   }
   void transform(void* out, int64_t bdist) override {
-    impl.transform(static_cast<Cplx<T>*>(out), bdist);
+    impl.transform(static_cast<Cplx<TC>*>(out), bdist);
   }
   void reset() override { impl.reset(); }
   void info(nnf_plan_info* i) override { impl.info(i); }
@@ -166,6 +174,17 @@ Base* make_holder(int precision, int dim, A&&... a) {
   if (dim == 1) return new H<double, 1>(std::forward<A>(a)...);
   if (dim == 2) return new H<double, 2>(std::forward<A>(a)...);
   return new H<double, 3>(std::forward<A>(a)...);
+}
+
+template <typename... A>
+PlanT3Iface* make_t3_holder(int precision, int dim, A&&... a) {
+  if (precision == NNF_MIXED) {
+    if (dim < 1 || dim > 3) throw neonufft::InputError("Neonufft:: dim must be 1, 2 or 3");
+    if (dim == 1) return new PlanT3Holder<float, 1, double>(std::forward<A>(a)...);
+    if (dim == 2) return new PlanT3Holder<float, 2, double>(std::forward<A>(a)...);
+    return new PlanT3Holder<float, 3, double>(std::forward<A>(a)...);
+  }
+  return make_holder<PlanT3Holder, PlanT3Iface>(precision, dim, std::forward<A>(a)...);
 }
 
 template <typename F>
@@ -295,6 +314,8 @@ NEONUFFT_EXPORT int nnf_plan_t3_grid_memory_size(int precision, int dim, const n
                                                  const double* output_min,
                                                  const double* output_max, uint64_t* size) {
   return guarded([&] {
+    // NNF_MIXED sizes its grids like NNF_F32 (they hold fp32 values)
+    if (precision == NNF_MIXED) precision = NNF_F32;
     check_kind(precision, dim);
     if (!size || !input_min || !input_max || !output_min || !output_max)
       throw neonufft::InputError("Neonufft:: null argument");
@@ -322,7 +343,7 @@ NEONUFFT_EXPORT int nnf_plan_t3_create(nnf_plan_t3* plan, int precision, int dim
     if (!plan) throw neonufft::InputError("Neonufft:: null plan output pointer");
     *plan = nullptr;
     auto h = std::make_unique<nnf_plan_t3_s>();
-    h->p.reset(make_holder<PlanT3Holder, PlanT3Iface>(
+    h->p.reset(make_t3_holder(
         precision, dim, to_options(opt), sign, static_cast<IntType>(num_in), input_points,
         static_cast<IntType>(num_out), output_points, static_cast<hipStream_t>(stream),
         static_cast<IntType>(batch_size), default_alloc()));
@@ -341,7 +362,7 @@ NEONUFFT_EXPORT int nnf_plan_t3_create_bounds(nnf_plan_t3* plan, int precision, 
     if (!input_min || !input_max || !output_min || !output_max)
       throw neonufft::InputError("Neonufft:: null bounds");
     auto h = std::make_unique<nnf_plan_t3_s>();
-    h->p.reset(make_holder<PlanT3Holder, PlanT3Iface>(
+    h->p.reset(make_t3_holder(
         precision, dim, to_options(opt), sign, input_min, input_max, output_min, output_max,
         static_cast<hipStream_t>(stream), static_cast<IntType>(batch_size), default_alloc()));
     *plan = h.release();
@@ -434,6 +455,21 @@ NEONUFFT_EXPORT int nnf_plan_fseries(nnf_plan plan, int d, const void** fs) {
     *fs = plan->p->fseries(d);
     if (!*fs) throw neonufft::InputError("Neonufft:: dimension out of range");
   });
+}
+
+NEONUFFT_EXPORT int nnf_autotune(int precision, int dim, const nnf_options* opt, int sign,
+                                 int64_t num_nu, const void* const* loc, const int64_t* modes,
+                                 void* stream, int types, int reps, int use_cache,
+                                 nnf_tuning* best, double* best_ms) {
+  return guarded([&] {
+    if (!modes || !best || !best_ms) throw neonufft::InputError("Neonufft:: null argument");
+    nnf::autotune(precision, dim, to_options(opt), sign, static_cast<IntType>(num_nu), loc, modes,
+                  static_cast<hipStream_t>(stream), types, reps, use_cache != 0, best, best_ms);
+  });
+}
+
+NEONUFFT_EXPORT int nnf_autotune_clear_cache(void) {
+  return guarded([&] { nnf::autotune_clear_cache(); });
 }
 
 NEONUFFT_EXPORT int nnf_batch_create(nnf_batch* batch, int precision, int dim,

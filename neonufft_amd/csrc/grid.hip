@@ -241,9 +241,9 @@ struct Vec3d {
   double v[3];
 };
 
-template <typename T, int DIM>
+template <typename TC, typename T, int DIM>
 __global__ void __launch_bounds__(kBlock)
-    prephase_kernel(int sign, Loc<T> loc, const uint32_t* __restrict__ perm, std::size_t m,
+    prephase_kernel(int sign, Loc<TC> loc, const uint32_t* __restrict__ perm, std::size_t m,
                     Vec3d sbar, Cplx<T>* __restrict__ pre) {
   for (std::size_t p = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock; p < m;
        p += static_cast<std::size_t>(gridDim.x) * kBlock) {
@@ -257,9 +257,9 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-template <typename T, int DIM>
+template <typename TC, typename T, int DIM>
 __global__ void __launch_bounds__(kBlock)
-    postphase_kernel(int sign, Loc<T> loc, const uint32_t* __restrict__ perm, std::size_t m,
+    postphase_kernel(int sign, Loc<TC> loc, const uint32_t* __restrict__ perm, std::size_t m,
                      Vec3d xbar, Vec3d sbar, Vec3d scale, QuadRule r, Cplx<T>* __restrict__ post) {
   for (std::size_t p = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock; p < m;
        p += static_cast<std::size_t>(gridDim.x) * kBlock) {
@@ -274,6 +274,17 @@ __global__ void __launch_bounds__(kBlock)
     double s, c;
     sincos(ph, &s, &c);
     post[p] = Cplx<T>{static_cast<T>(c / phihat), static_cast<T>(sign * s / phihat)};
+  }
+}
+
+// Complex precision conversion (mixed-precision type-3 I/O), grid-stride.
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(kBlock)
+    convert_kernel(const Cplx<TI>* __restrict__ in, Cplx<TO>* __restrict__ out, std::size_t m) {
+  for (std::size_t i = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock; i < m;
+       i += static_cast<std::size_t>(gridDim.x) * kBlock) {
+    const Cplx<TI> v = in[i];
+    out[i] = Cplx<TO>{static_cast<TO>(v.x), static_cast<TO>(v.y)};
   }
 }
 
@@ -390,19 +401,19 @@ void launch_downsample_batch(int order_fft, const Cplx<T>* grids, std::size_t gs
   check_launch("downsample_batch_kernel");
 }
 
-template <typename T, int DIM>
-void launch_prephase(int sign, Loc<T> loc, const uint32_t* perm, std::size_t m,
+template <typename TC, typename T, int DIM>
+void launch_prephase(int sign, Loc<TC> loc, const uint32_t* perm, std::size_t m,
                      const double* sbar, Cplx<T>* pre_sorted, hipStream_t stream) {
   if (m == 0) return;
   Vec3d sb{{0, 0, 0}};
   for (int d = 0; d < DIM; ++d) sb.v[d] = sbar[d];
-  hipLaunchKernelGGL((prephase_kernel<T, DIM>), dim3(blocks_for(m)), dim3(kBlock), 0, stream, sign,
+  hipLaunchKernelGGL((prephase_kernel<TC, T, DIM>), dim3(blocks_for(m)), dim3(kBlock), 0, stream, sign,
                      loc, perm, m, sb, pre_sorted);
   check_launch("prephase_kernel");
 }
 
-template <typename T, int DIM>
-void launch_postphase(int sign, Loc<T> loc, const uint32_t* perm, std::size_t m,
+template <typename TC, typename T, int DIM>
+void launch_postphase(int sign, Loc<TC> loc, const uint32_t* perm, std::size_t m,
                       const double* xbar, const double* sbar, const double* scale,
                       const QuadRule& rule, Cplx<T>* post_sorted, hipStream_t stream) {
   if (m == 0) return;
@@ -412,10 +423,23 @@ void launch_postphase(int sign, Loc<T> loc, const uint32_t* perm, std::size_t m,
     sb.v[d] = sbar[d];
     sc.v[d] = scale[d];
   }
-  hipLaunchKernelGGL((postphase_kernel<T, DIM>), dim3(blocks_for(m)), dim3(kBlock), 0, stream,
+  hipLaunchKernelGGL((postphase_kernel<TC, T, DIM>), dim3(blocks_for(m)), dim3(kBlock), 0, stream,
                      sign, loc, perm, m, xb, sb, sc, rule, post_sorted);
   check_launch("postphase_kernel");
 }
+
+template <typename TI, typename TO>
+void launch_convert(const Cplx<TI>* in, Cplx<TO>* out, std::size_t m, hipStream_t stream) {
+  if (m == 0) return;
+  hipLaunchKernelGGL((convert_kernel<TI, TO>), dim3(blocks_for(m)), dim3(kBlock), 0, stream, in,
+                     out, m);
+  check_launch("convert_kernel");
+}
+
+template void launch_convert<double, float>(const Cplx<double>*, Cplx<float>*, std::size_t,
+                                            hipStream_t);
+template void launch_convert<float, double>(const Cplx<float>*, Cplx<double>*, std::size_t,
+                                            hipStream_t);
 
 #define NNF_INST_GRID_T(T)                                                                   \
   template void launch_fill_zero<T>(Cplx<T>*, std::size_t, hipStream_t);                    \
@@ -426,11 +450,11 @@ void launch_postphase(int sign, Loc<T> loc, const uint32_t* perm, std::size_t m,
                                         T* const*, Cplx<T>*, const IntType*, hipStream_t);        \
   template void launch_downsample<T, DIM>(int, const Cplx<T>*, const IntType*, T* const*,         \
                                           Cplx<T>*, const IntType*, const IntType*, hipStream_t); \
-  template void launch_prephase<T, DIM>(int, Loc<T>, const uint32_t*, std::size_t,                \
-                                        const double*, Cplx<T>*, hipStream_t);                    \
-  template void launch_postphase<T, DIM>(int, Loc<T>, const uint32_t*, std::size_t,               \
-                                         const double*, const double*, const double*,             \
-                                         const QuadRule&, Cplx<T>*, hipStream_t);                 \
+  template void launch_prephase<T, T, DIM>(int, Loc<T>, const uint32_t*, std::size_t,             \
+                                           const double*, Cplx<T>*, hipStream_t);                 \
+  template void launch_postphase<T, T, DIM>(int, Loc<T>, const uint32_t*, std::size_t,            \
+                                            const double*, const double*, const double*,          \
+                                            const QuadRule&, Cplx<T>*, hipStream_t);              \
   template void launch_upsample_batch<T, DIM>(int, const Cplx<T>* const*, const IntType*,         \
                                               const IntType*, T* const*, Cplx<T>*, std::size_t,   \
                                               const IntType*, int, hipStream_t);                  \
@@ -446,5 +470,17 @@ NNF_INST_GRID(float, 3)
 NNF_INST_GRID(double, 1)
 NNF_INST_GRID(double, 2)
 NNF_INST_GRID(double, 3)
+
+// double coordinates, fp32 phases (mixed-precision type 3)
+#define NNF_INST_PHASE_MIXED(DIM)                                                            \
+  template void launch_prephase<double, float, DIM>(int, Loc<double>, const uint32_t*,       \
+                                                    std::size_t, const double*, Cplx<float>*, \
+                                                    hipStream_t);                            \
+  template void launch_postphase<double, float, DIM>(                                        \
+      int, Loc<double>, const uint32_t*, std::size_t, const double*, const double*,          \
+      const double*, const QuadRule&, Cplx<float>*, hipStream_t);
+NNF_INST_PHASE_MIXED(1)
+NNF_INST_PHASE_MIXED(2)
+NNF_INST_PHASE_MIXED(3)
 
 }  // namespace nnf

@@ -26,14 +26,16 @@ struct QuadRule {
 constexpr uint32_t kMaxSubproblemPoints = 2048;
 
 // ---- points.hip: fold / rescale, counting sort by bin, subproblems -------
-template <typename T, int DIM>
-void launch_bin_count(Loc<T> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
+template <typename TC, int DIM>
+void launch_bin_count(Loc<TC> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
                       uint32_t* count, uint32_t* rank, hipStream_t stream);
 
-template <typename T, int DIM>
-void launch_bin_scatter(Loc<T> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
-                        const uint32_t* bin_start, const uint32_t* rank, PointRec<T, DIM>* pts,
-                        uint32_t* perm, hipStream_t stream);
+// TC: coordinate type read; T: precision of the point records written
+// (TC = T, or TC = double with T = float for mixed-precision type 3).
+template <typename TC, typename T, int DIM>
+void launch_bin_scatter(Loc<TC> loc, std::size_t m, const FoldParams& fp, const BinGeom& g,
+                        int ns, const uint32_t* bin_start, const uint32_t* rank,
+                        PointRec<T, DIM>* pts, uint32_t* perm, hipStream_t stream);
 
 // Exclusive prefix sum of n uint32 values; `scratch` must hold
 // scan_scratch_size(n) values.
@@ -144,13 +146,18 @@ void launch_subprob_retag(const Subprob* src, std::size_t nsub, uint32_t job, Su
 
 // Type-3 phase factors, written in bin (sorted) order: value[p] belongs to
 // point perm[p].
-template <typename T, int DIM>
-void launch_prephase(int sign, Loc<T> loc, const uint32_t* perm, std::size_t m,
+// TC: coordinate type read (phase arguments are formed in FP64 either way).
+template <typename TC, typename T, int DIM>
+void launch_prephase(int sign, Loc<TC> loc, const uint32_t* perm, std::size_t m,
                      const double* sbar, Cplx<T>* pre_sorted, hipStream_t stream);
 
-template <typename T, int DIM>
-void launch_postphase(int sign, Loc<T> loc, const uint32_t* perm, std::size_t m,
+template <typename TC, typename T, int DIM>
+void launch_postphase(int sign, Loc<TC> loc, const uint32_t* perm, std::size_t m,
                       const double* xbar, const double* sbar, const double* scale,
                       const QuadRule& rule, Cplx<T>* post_sorted, hipStream_t stream);
+
+// out[i] = in[i] converted between complex precisions.
+template <typename TI, typename TO>
+void launch_convert(const Cplx<TI>* in, Cplx<TO>* out, std::size_t m, hipStream_t stream);
 
 }  // namespace nnf

@@ -140,8 +140,14 @@ private:
   PointSet<T, DIM> pts_;
 };
 
-template <typename T, int DIM>
+// T: precision of spreading, FFT and interpolation. TC: precision of the
+// coordinates and of the complex inputs / outputs; TC = double with T = float
+// is the mixed-precision plan (FP64 fold and phases from double coordinates,
+// complex128 I/O converted at the plan's edges, fp32 bulk work).
+template <typename T, int DIM, typename TC = T>
 class PlanT3Impl {
+  static constexpr bool kMixed = !std::is_same_v<T, TC>;
+
 public:
   // sizeof(complex T) * (spread grid + 2 * FFT grid), the reference's formula
   // (gpu/plan_t3_impl.hpp:54-72).
@@ -161,8 +167,8 @@ public:
     return sizeof(Cplx<T>) * (sg + 2 * fg);
   }
 
-  PlanT3Impl(const neonufft::Options& opt, int sign, IntType num_in, const T* const* in_pts,
-             IntType num_out, const T* const* out_pts, hipStream_t stream, IntType batch,
+  PlanT3Impl(const neonufft::Options& opt, int sign, IntType num_in, const TC* const* in_pts,
+             IntType num_out, const TC* const* out_pts, hipStream_t stream, IntType batch,
              std::shared_ptr<neonufft::Allocator> alloc, const PlanTuning& tune = PlanTuning())
       : opt_(opt), sign_(sign), stream_(stream), batch_(batch), alloc_(std::move(alloc)), tune_(tune) {
     check_common();
@@ -183,9 +189,9 @@ public:
     init(in_min, in_max, out_min, out_max);
   }
 
-  void set_input_points(IntType num_in, const T* const* pts) {
+  void set_input_points(IntType num_in, const TC* const* pts) {
     if (num_in < 0) throw neonufft::InputError("Neonufft:: negative point count");
-    Loc<T> l = make_loc(pts);
+    Loc<TC> l = make_loc(pts);
     FoldParams fp{};
     for (int d = 0; d < 3; ++d) {
       fp.offset[d] = d < DIM ? gi_.input_offsets[d] : 0.0;
@@ -197,14 +203,15 @@ public:
     for (int d = 0; d < DIM; ++d) has_pre_ = has_pre_ || gi_.output_offsets[d] != 0.0;
     if (has_pre_) {
       pre_.ensure(num_in ? static_cast<std::size_t>(num_in) : 1, alloc_);
-      launch_prephase<T, DIM>(sign_, l, in_pts_.perm(), static_cast<std::size_t>(num_in),
-                              gi_.output_offsets, pre_.data(), stream_);
+      launch_prephase<TC, T, DIM>(sign_, l, in_pts_.perm(), static_cast<std::size_t>(num_in),
+                                  gi_.output_offsets, pre_.data(), stream_);
     }
+    if constexpr (kMixed) io_in_.ensure(num_in ? static_cast<std::size_t>(num_in) : 1, alloc_);
   }
 
-  void set_output_points(IntType num_out, const T* const* pts) {
+  void set_output_points(IntType num_out, const TC* const* pts) {
     if (num_out < 0) throw neonufft::InputError("Neonufft:: negative point count");
-    Loc<T> l = make_loc(pts);
+    Loc<TC> l = make_loc(pts);
     FoldParams fp{};
     double scale[3] = {1, 1, 1};
     for (int d = 0; d < 3; ++d) {
@@ -215,22 +222,30 @@ public:
     out_pts_.build(l, static_cast<std::size_t>(num_out), fp, geom_out_, ks_.ns, tune_.max_points,
                    stream_, alloc_);
     post_.ensure(num_out ? static_cast<std::size_t>(num_out) : 1, alloc_);
-    launch_postphase<T, DIM>(sign_, l, out_pts_.perm(), static_cast<std::size_t>(num_out),
-                             gi_.input_offsets, gi_.output_offsets, scale, ks_.quad, post_.data(),
-                             stream_);
+    launch_postphase<TC, T, DIM>(sign_, l, out_pts_.perm(), static_cast<std::size_t>(num_out),
+                                 gi_.input_offsets, gi_.output_offsets, scale, ks_.quad,
+                                 post_.data(), stream_);
+    if constexpr (kMixed) io_out_.ensure(num_out ? static_cast<std::size_t>(num_out) : 1, alloc_);
   }
 
-  void add_input(const Cplx<T>* in, IntType bdist) {
+  void add_input(const Cplx<TC>* in, IntType bdist) {
     if (bdist <= 0) bdist = static_cast<IntType>(in_pts_.size());
     if (in_pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
     for (IntType b = 0; b < batch_; ++b) {
+      const Cplx<T>* src;
+      if constexpr (kMixed) {
+        launch_convert<TC, T>(in + b * bdist, io_in_.data(), in_pts_.size(), stream_);
+        src = io_in_.data();
+      } else {
+        src = in + b * bdist;
+      }
       launch_spread<T, DIM>(ks_.ns, in_pts_.recs(), in_pts_.perm(), in_pts_.subs(), in_pts_.nsub(),
-                            in + b * bdist, has_pre_ ? pre_.data() : nullptr,
+                            src, has_pre_ ? pre_.data() : nullptr,
                             spread_grid_.data() + b * spread_cells_, geom_in_, ks_.es, stream_);
     }
   }
 
-  void transform(Cplx<T>* out, IntType bdist) {
+  void transform(Cplx<TC>* out, IntType bdist) {
     if (bdist <= 0) bdist = static_cast<IntType>(out_pts_.size());
     if (out_pts_.size() && !out) throw neonufft::InputError("Neonufft:: null output");
     T* fs[3] = {fs_[0].data(), fs_[1].data(), fs_[2].data()};
@@ -242,9 +257,18 @@ public:
                               gi_.spread_grid_size, fs, fft_grid_.data(), gi_.fft_grid_size,
                               stream_);
       fft_.execute(fft_grid_.data());
+      Cplx<T>* dst;
+      if constexpr (kMixed) {
+        dst = io_out_.data();
+      } else {
+        dst = out + b * bdist;
+      }
       launch_interp<T, DIM>(ks_.ns, out_pts_.recs(), out_pts_.perm(), out_pts_.subs(),
-                            out_pts_.nsub(), fft_grid_.data(), post_.data(), out + b * bdist,
-                            geom_out_, ks_.es, stream_);
+                            out_pts_.nsub(), fft_grid_.data(), post_.data(), dst, geom_out_,
+                            ks_.es, stream_);
+      if constexpr (kMixed) {
+        launch_convert<T, TC>(io_out_.data(), out + b * bdist, out_pts_.size(), stream_);
+      }
     }
   }
 
@@ -254,7 +278,7 @@ public:
 
   void info(nnf_plan_info* info) const {
     *info = nnf_plan_info{};
-    info->precision = std::is_same_v<T, float> ? NNF_F32 : NNF_F64;
+    info->precision = kMixed ? NNF_MIXED : (std::is_same_v<T, float> ? NNF_F32 : NNF_F64);
     info->dim = DIM;
     info->n_spread = ks_.ns;
     info->es_beta = ks_.beta;
@@ -281,8 +305,8 @@ private:
     ks_ = make_kernel_setup(opt_.tol, opt_.upsampfac, std::is_same_v<T, float>);
   }
 
-  static Loc<T> make_loc(const T* const* pts) {
-    Loc<T> l{{nullptr, nullptr, nullptr}};
+  static Loc<TC> make_loc(const TC* const* pts) {
+    Loc<TC> l{{nullptr, nullptr, nullptr}};
     for (int d = 0; d < DIM; ++d) l.p[d] = pts ? pts[d] : nullptr;
     return l;
   }
@@ -290,7 +314,7 @@ private:
   // Per-dimension min/max of device coordinates, finished on the host
   // (the reference does the same with hipCUB + a blocking copy,
   // gpu/plan_t3_impl.hpp:88-102).
-  void minmax(IntType m, const T* const* pts, double* lo, double* hi) {
+  void minmax(IntType m, const TC* const* pts, double* lo, double* hi) {
     for (int d = 0; d < 3; ++d) {
       lo[d] = 0.0;
       hi[d] = 0.0;
@@ -300,7 +324,7 @@ private:
     DeviceBuffer<double> part(static_cast<std::size_t>(2 * nblk * DIM), alloc_);
     for (int d = 0; d < DIM; ++d) {
       if (!pts || !pts[d]) throw neonufft::InputError("Neonufft:: null coordinate pointer");
-      launch_minmax<T>(pts[d], static_cast<std::size_t>(m), part.data() + 2 * nblk * d, stream_);
+      launch_minmax<TC>(pts[d], static_cast<std::size_t>(m), part.data() + 2 * nblk * d, stream_);
     }
     std::vector<double> host(static_cast<std::size_t>(2 * nblk * DIM));
     check_hip(hipMemcpyAsync(host.data(), part.data(), host.size() * sizeof(double),
@@ -352,6 +376,7 @@ private:
   IntType spread_cells_ = 1;
   BinGeom geom_in_{}, geom_out_{};
   DeviceBuffer<Cplx<T>> spread_grid_, fft_grid_, pre_, post_;
+  DeviceBuffer<Cplx<T>> io_in_, io_out_;  // mixed precision: converted inputs / outputs
   bool has_pre_ = false;
   FftPlan fft_;
   DeviceBuffer<T> fs_[3];

@@ -13,7 +13,8 @@ class PointSet {
 public:
   // Folds, bins and sorts m points (loc[d][i]); synchronises `stream` once to
   // size the subproblem table.
-  void build(Loc<T> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
+  template <typename TC>
+  void build(Loc<TC> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
              uint32_t max_points, hipStream_t stream, const std::shared_ptr<neonufft::Allocator>& alloc) {
     if (m > 0xffffffffull) {
       throw neonufft::NotImplementedError("Neonufft:: more than 2^32 - 1 points per plan");
@@ -41,7 +42,7 @@ public:
               "hipMemsetAsync(bin counts)");
     check_hip(hipMemsetAsync(nsubb_.data(), 0, (nbins + 1) * sizeof(uint32_t), stream),
               "hipMemsetAsync(subproblem counts)");
-    launch_bin_count<T, DIM>(loc, m, fp, g, ns, count_.data(), rank_.data(), stream);
+    launch_bin_count<TC, DIM>(loc, m, fp, g, ns, count_.data(), rank_.data(), stream);
     launch_exclusive_scan(count_.data(), start_.data(), nbins + 1, scratch_.data(), stream);
     launch_subprob_counts(count_.data(), nbins, max_points, nsubb_.data(), stream);
     launch_exclusive_scan(nsubb_.data(), sub_start_.data(), nbins + 1, scratch_.data(), stream);
@@ -54,7 +55,7 @@ public:
     subs_.ensure(nsub_ ? nsub_ : 1, alloc);
     launch_subprob_build(count_.data(), start_.data(), sub_start_.data(), nbins, max_points,
                          subs_.data(), stream);
-    launch_bin_scatter<T, DIM>(loc, m, fp, g, ns, start_.data(), rank_.data(), recs_.data(),
+    launch_bin_scatter<TC, T, DIM>(loc, m, fp, g, ns, start_.data(), rank_.data(), recs_.data(),
                                perm_.data(), stream);
     launch_bank_order<T, DIM>(subs_.data(), nsub_, recs_.data(), perm_.data(), g, stream);
   }

@@ -38,8 +38,8 @@ __device__ inline void fold_coord(double v, int n, int ns, double& hx, int& i0, 
   cell = min(static_cast<int>(floor(xi)), n - 1);
 }
 
-template <typename T, int DIM>
-__device__ inline uint32_t point_bin(const Loc<T>& loc, std::size_t i, const FoldParams& fp,
+template <typename TC, int DIM>
+__device__ inline uint32_t point_bin(const Loc<TC>& loc, std::size_t i, const FoldParams& fp,
                                      const BinGeom& g, int ns, double (&hx)[3], int (&rel)[3]) {
   uint32_t bin = 0, mult = 1;
 #pragma unroll
@@ -55,29 +55,30 @@ __device__ inline uint32_t point_bin(const Loc<T>& loc, std::size_t i, const Fol
   return bin;
 }
 
-template <typename T, int DIM>
+template <typename TC, int DIM>
 __global__ void __launch_bounds__(kBlock)
-    bin_count_kernel(Loc<T> loc, std::size_t m, FoldParams fp, BinGeom g, int ns,
+    bin_count_kernel(Loc<TC> loc, std::size_t m, FoldParams fp, BinGeom g, int ns,
                      uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
   for (std::size_t i = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock; i < m;
        i += static_cast<std::size_t>(gridDim.x) * kBlock) {
     double hx[3];
     int rel[3];
-    const uint32_t bin = point_bin<T, DIM>(loc, i, fp, g, ns, hx, rel);
+    const uint32_t bin = point_bin<TC, DIM>(loc, i, fp, g, ns, hx, rel);
     rank[i] = atomicAdd(&count[bin], 1u);
   }
 }
 
-template <typename T, int DIM>
+// TC: coordinate type read; T: precision of the point records written.
+template <typename TC, typename T, int DIM>
 __global__ void __launch_bounds__(kBlock)
-    bin_scatter_kernel(Loc<T> loc, std::size_t m, FoldParams fp, BinGeom g, int ns,
+    bin_scatter_kernel(Loc<TC> loc, std::size_t m, FoldParams fp, BinGeom g, int ns,
                        const uint32_t* __restrict__ bin_start, const uint32_t* __restrict__ rank,
                        PointRec<T, DIM>* __restrict__ pts, uint32_t* __restrict__ perm) {
   for (std::size_t i = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock; i < m;
        i += static_cast<std::size_t>(gridDim.x) * kBlock) {
     double hx[3];
     int rel[3];
-    const uint32_t bin = point_bin<T, DIM>(loc, i, fp, g, ns, hx, rel);
+    const uint32_t bin = point_bin<TC, DIM>(loc, i, fp, g, ns, hx, rel);
     PointRec<T, DIM> rec{};
     uint32_t cell = 0;
 #pragma unroll
@@ -312,21 +313,21 @@ __global__ void __launch_bounds__(kBlock)
 
 }  // namespace
 
-template <typename T, int DIM>
-void launch_bin_count(Loc<T> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
+template <typename TC, int DIM>
+void launch_bin_count(Loc<TC> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
                       uint32_t* count, uint32_t* rank, hipStream_t stream) {
   if (m == 0) return;
-  hipLaunchKernelGGL((bin_count_kernel<T, DIM>), dim3(grid_for(m, kBlock)), dim3(kBlock), 0,
+  hipLaunchKernelGGL((bin_count_kernel<TC, DIM>), dim3(grid_for(m, kBlock)), dim3(kBlock), 0,
                      stream, loc, m, fp, g, ns, count, rank);
   check_launch("bin_count_kernel");
 }
 
-template <typename T, int DIM>
-void launch_bin_scatter(Loc<T> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
-                        const uint32_t* bin_start, const uint32_t* rank, PointRec<T, DIM>* pts,
-                        uint32_t* perm, hipStream_t stream) {
+template <typename TC, typename T, int DIM>
+void launch_bin_scatter(Loc<TC> loc, std::size_t m, const FoldParams& fp, const BinGeom& g,
+                        int ns, const uint32_t* bin_start, const uint32_t* rank,
+                        PointRec<T, DIM>* pts, uint32_t* perm, hipStream_t stream) {
   if (m == 0) return;
-  hipLaunchKernelGGL((bin_scatter_kernel<T, DIM>), dim3(grid_for(m, kBlock)), dim3(kBlock), 0,
+  hipLaunchKernelGGL((bin_scatter_kernel<TC, T, DIM>), dim3(grid_for(m, kBlock)), dim3(kBlock), 0,
                      stream, loc, m, fp, g, ns, bin_start, rank, pts, perm);
   check_launch("bin_scatter_kernel");
 }
@@ -396,10 +397,10 @@ void launch_minmax(const T* x, std::size_t m, double* partial, hipStream_t strea
 #define NNF_INST_POINTS(T, DIM)                                                                 \
   template void launch_bin_count<T, DIM>(Loc<T>, std::size_t, const FoldParams&, const BinGeom&, \
                                          int, uint32_t*, uint32_t*, hipStream_t);               \
-  template void launch_bin_scatter<T, DIM>(Loc<T>, std::size_t, const FoldParams&,              \
-                                           const BinGeom&, int, const uint32_t*,                \
-                                           const uint32_t*, PointRec<T, DIM>*, uint32_t*,       \
-                                           hipStream_t);                                        \
+  template void launch_bin_scatter<T, T, DIM>(Loc<T>, std::size_t, const FoldParams&,           \
+                                              const BinGeom&, int, const uint32_t*,             \
+                                              const uint32_t*, PointRec<T, DIM>*, uint32_t*,    \
+                                              hipStream_t);                                     \
   template void launch_bank_order<T, DIM>(const Subprob*, std::size_t, PointRec<T, DIM>*,       \
                                           uint32_t*, const BinGeom&, hipStream_t);
 
@@ -409,6 +410,20 @@ NNF_INST_POINTS(float, 3)
 NNF_INST_POINTS(double, 1)
 NNF_INST_POINTS(double, 2)
 NNF_INST_POINTS(double, 3)
+
+// double coordinates binned for fp32 records (mixed-precision type 3)
+template void launch_bin_scatter<double, float, 1>(Loc<double>, std::size_t, const FoldParams&,
+                                                   const BinGeom&, int, const uint32_t*,
+                                                   const uint32_t*, PointRec<float, 1>*,
+                                                   uint32_t*, hipStream_t);
+template void launch_bin_scatter<double, float, 2>(Loc<double>, std::size_t, const FoldParams&,
+                                                   const BinGeom&, int, const uint32_t*,
+                                                   const uint32_t*, PointRec<float, 2>*,
+                                                   uint32_t*, hipStream_t);
+template void launch_bin_scatter<double, float, 3>(Loc<double>, std::size_t, const FoldParams&,
+                                                   const BinGeom&, int, const uint32_t*,
+                                                   const uint32_t*, PointRec<float, 3>*,
+                                                   uint32_t*, hipStream_t);
 
 template void launch_minmax<float>(const float*, std::size_t, double*, hipStream_t);
 template void launch_minmax<double>(const double*, std::size_t, double*, hipStream_t);

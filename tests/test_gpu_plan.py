@@ -207,3 +207,44 @@ def test_type3(case, prec):
     lim = 1.2 * alg_err + (1e-12 if prec == "f64" else 2e-5)
     assert err < lim, (err, alg_err)
     assert O.rel_l2(got, alg) < (1e-9 if prec == "f64" else 5e-5 + 0.05 * tol)
+
+
+T3_MIXED_CASES = T3_CASES + [
+    # large-magnitude sources and small frequencies, the layout where fp32
+    # coordinates alone would cost ~1e-5 of phase (|x| ~ 1e4, |x.s| ~ 200)
+    ("wide", 3000, 700, 1e-6, -1),
+]
+
+
+@pytest.mark.parametrize("case", T3_MIXED_CASES,
+                         ids=[f"d{c[0]}_m{c[1]}_n{c[2]}_tol{c[3]:g}_s{c[4]}" for c in T3_MIXED_CASES])
+def test_type3_mixed(case):
+    """NNF_MIXED type 3 (extension): double coordinates and complex128 I/O with
+    fp32 spreading / FFT / interpolation. Error bound as for fp32 plans, but
+    against the direct sum over the unrounded double coordinates."""
+    dim, m, n, tol, sign = case
+    rng = np.random.default_rng(4242 + m + n)
+    if dim == "wide":
+        dim = 3
+        x = [rng.uniform(-1e4, 1e4, m), rng.uniform(-1e4, 1e4, m), rng.uniform(-50, 50, m)]
+        s = [rng.uniform(-0.02, 0.02, n), rng.uniform(-0.02, 0.02, n), rng.uniform(-1, 1, n)]
+    else:
+        x = points(rng, dim, m, "f64")
+        s = points(rng, dim, n, "f64")
+    c = cplx(rng, m, "f64")
+    opt = nnf.Options(tol=tol)
+    plan = nnf.PlanT3(opt, sign, m, [dev(v, torch.float64) for v in x], n,
+                      [dev(v, torch.float64) for v in s], dtype="mixed")
+    assert plan.info()["precision"] == "mixed"
+    out = torch.zeros(n, dtype=torch.complex128, device="cuda")
+    plan.add_input(dev(c, torch.complex128))
+    plan.transform(out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    ref = O.direct_type3(x, c, s, sign)
+    err = O.rel_l2(got, ref)
+    alg = O.nufft3(x, c, s, sign, tol)
+    alg_err = O.rel_l2(alg, ref)
+    assert alg_err < 20 * tol
+    assert err < 1.2 * alg_err + 2e-5, (err, alg_err)
+    assert O.rel_l2(got, alg) < 5e-5 + 0.05 * tol
