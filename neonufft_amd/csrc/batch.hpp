@@ -19,6 +19,7 @@
 #include <memory>
 #include <vector>
 
+#include "cover.hpp"
 #include "plan_impl.hpp"
 
 namespace nnf {
@@ -120,11 +121,14 @@ public:
       it.group = f->second;
       groups_[it.group].members.push_back(t);
     }
-    // lay the grids out group by group
+    // lay the grids out group by group, each group starting on a 256-byte
+    // boundary (the up/downsampling kernels store cell pairs as 16 bytes)
     std::size_t off = 0;
     for (Group& g : groups_) {
       g.cells = 1;
       for (int d = 0; d < DIM; ++d) g.cells *= static_cast<std::size_t>(g.n[d]);
+      constexpr std::size_t kAlignCells = 256 / sizeof(Cplx<T>);
+      off = (off + kAlignCells - 1) / kAlignCells * kAlignCells;
       g.offset = off;
       for (std::size_t k = 0; k < g.members.size(); ++k) {
         items_[g.members[k]].grid_off = off + k * g.cells;
@@ -140,7 +144,13 @@ public:
         launch_fseries<T>(g.fs[d].data(), g.n[d], ks_.quad, stream_);
       }
       g.geom = choose_bins(DIM, g.n, ks_.ns, sizeof(Cplx<T>), tune_.bin);
+      g.cover.build(DIM, g.geom, ks_.ns, stream_, alloc_);
       max_lds_ = std::max(max_lds_, tile_bytes<T>(g.geom));
+      max_n0_ = std::max(max_n0_, g.geom.n[0]);
+      max_rows_ = std::max(max_rows_, g.geom.n[1] * g.geom.n[2]);
+      max_bins_ = std::max<std::size_t>(
+          max_bins_, static_cast<std::size_t>(g.geom.nbins[0]) * g.geom.nbins[1] * g.geom.nbins[2]);
+      regular_ = regular_ && bins_regular(DIM, g.geom, ks_.ns);
     }
     std::size_t io_off = 0;
     for (Group& g : groups_) {
@@ -150,6 +160,8 @@ public:
     pts_.resize(count_);
     sjobs_.reset(count_, alloc_);
     ijobs_.reset(count_, alloc_);
+    rjobs_.reset(count_, alloc_);
+    host_rjobs_.resize(count_);
     io_.reset(count_, alloc_);
     host_sjobs_.resize(count_);
     host_ijobs_.resize(count_);
@@ -173,7 +185,7 @@ public:
   void set_points(const IntType* num_nu, const T* const* loc) {
     if (!num_nu) throw neonufft::InputError("Neonufft:: null point counts");
     drop_graphs();
-    std::size_t total_sub = 0;
+    std::size_t total_sub = 0, total_tiles = 0;
     total_points_ = 0;
     for (int t = 0; t < count_; ++t) {
       if (num_nu[t] < 0) throw neonufft::InputError("Neonufft:: negative number of points");
@@ -183,9 +195,21 @@ public:
       pts_[t].build(l, static_cast<std::size_t>(num_nu[t]), fp, groups_[items_[t].group].geom,
                     ks_.ns, tune_.max_points, stream_, alloc_);
       items_[t].sub_off = total_sub;
+      items_[t].tiles_off = total_tiles;
       total_sub += pts_[t].nsub();
+      total_tiles += pts_[t].nsub() * tile_cells(groups_[items_[t].group].geom);
       total_points_ += static_cast<std::size_t>(num_nu[t]);
     }
+    tiles_.ensure(total_tiles ? total_tiles : 1, alloc_);
+    for (int t = 0; t < count_; ++t) {
+      const Item& it = items_[t];
+      const Group& g = groups_[it.group];
+      host_rjobs_[t] = ReduceJob<T, DIM>{tiles_.data() + it.tiles_off, pts_[t].sub_starts(),
+                                         grids_.data() + it.grid_off, g.geom, g.cover.table()};
+    }
+    check_hip(hipMemcpyAsync(rjobs_.data(), host_rjobs_.data(),
+                             sizeof(ReduceJob<T, DIM>) * count_, hipMemcpyHostToDevice, stream_),
+              "hipMemcpyAsync(reduce jobs)");
     subs_.ensure(total_sub ? total_sub : 1, alloc_);
     up_sjobs_.invalidate();
     up_ijobs_.invalidate();
@@ -203,7 +227,9 @@ public:
     for (int t = 0; t < count_; ++t) {
       const Item& it = items_[t];
       host_sjobs_[t] = SpreadJob<T, DIM>{pts_[t].recs(), pts_[t].perm(), in[t], nullptr,
-                                         grids_.data() + it.grid_off, groups_[it.group].geom};
+                                         tiles_.data() + it.tiles_off,
+                                         static_cast<uint32_t>(it.sub_off), 0u,
+                                         groups_[it.group].geom};
     }
     up_sjobs_.upload(sjobs_.data(), host_sjobs_.data(), stream_);
     upload_io(reinterpret_cast<void* const*>(out));
@@ -237,6 +263,7 @@ private:
     IntType n[3];
     std::size_t grid_off = 0;
     std::size_t sub_off = 0;
+    std::size_t tiles_off = 0;  // first tile cell of this transform in tiles_
     int group = 0;
   };
   struct Group {
@@ -247,6 +274,7 @@ private:
     FftPlan fft;
     DeviceBuffer<T> fs[3];
     BinGeom geom{};
+    CoverBuffers cover;
     std::size_t io_off = 0;  // first entry of this group's caller pointers in io_
   };
 
@@ -263,8 +291,10 @@ private:
 
   void enqueue(int type, hipStream_t s) {
     if (type == 1) {
-      launch_fill_zero<T>(grids_.data(), grids_.size(), s);
       launch_spread_batch<T, DIM>(ks_.ns, sjobs_.data(), subs_.data(), nsub_, max_lds_, ks_.es, s);
+      // every fine-grid cell is written, so the grids need no zero fill
+      launch_tile_reduce_batch<T, DIM>(rjobs_.data(), count_, ks_.ns, regular_, max_bins_,
+                                       max_n0_, max_rows_, false, s);
       for (Group& g : groups_) {
         g.fft.set_stream(s);
         g.fft.execute(grids_.data() + g.offset);
@@ -349,6 +379,12 @@ private:
   std::vector<PointSet<T, DIM>> pts_;
   DeviceBuffer<Subprob> subs_;
   std::size_t nsub_ = 0, total_points_ = 0, max_lds_ = 0;
+  int max_n0_ = 0, max_rows_ = 0;
+  std::size_t max_bins_ = 0;
+  bool regular_ = true;  // every group's bins suit the bin-owned reduction
+  DeviceBuffer<Cplx<T>> tiles_;  // spreading tiles of all transforms
+  DeviceBuffer<ReduceJob<T, DIM>> rjobs_;
+  std::vector<ReduceJob<T, DIM>> host_rjobs_;
   DeviceBuffer<SpreadJob<T, DIM>> sjobs_;
   DeviceBuffer<InterpJob<T, DIM>> ijobs_;
   DeviceBuffer<void*> io_;  // outputs of type 1, inputs of type 2

@@ -22,9 +22,6 @@ struct FsPtrs {
   const T* p[3];
 };
 
-struct Dims3 {
-  IntType v[3];
-};
 
 // Zeroes n4 16-byte units and, if tail8 is set, one trailing 8-byte unit.
 __global__ void __launch_bounds__(kBlock)
@@ -52,15 +49,22 @@ __global__ void __launch_bounds__(kBlock) fseries_kernel(T* __restrict__ fs, Int
   }
 }
 
-// Mode k of an N-mode axis is in [-N/2, N-1-N/2]. Fine-grid cell g holds mode
-// k = g (g < N - N/2) or k = g - n (g >= n - N/2); other cells are padding.
-__device__ inline bool cell_mode(IntType g, IntType n, IntType N, IntType& k) {
-  const IntType nneg = N / 2;
-  if (g < N - nneg) {
+// Up/down-sampling geometry in 32-bit per-axis form: fine cells n, modes N,
+// mode-array element strides st (unused axes: n = N = 1).
+struct SampleGeom {
+  int n[3];
+  int N[3];
+  long long st[3];
+};
+
+// Mode of fine cell g on one axis (k = g for g < N - N/2, k = g - n for
+// g >= n - N/2), or false for a padding cell.
+__device__ inline bool cell_mode32(int g, int n, int N, int& k) {
+  if (g < N - N / 2) {
     k = g;
     return true;
   }
-  if (g >= n - nneg) {
+  if (g >= n - N / 2) {
     k = g - n;
     return true;
   }
@@ -68,173 +72,139 @@ __device__ inline bool cell_mode(IntType g, IntType n, IntType N, IntType& k) {
 }
 
 // Array index of mode k (CMCL: k + N/2; FFT order: k, negatives after).
-__device__ inline IntType mode_index(IntType k, IntType N, int order_fft) {
-  if (order_fft) return k >= 0 ? k : k + N;
-  return k + N / 2;
+__device__ inline int mode_index32(int k, int N, int order_fft) {
+  return order_fft ? (k >= 0 ? k : k + N) : k + N / 2;
 }
 
+// Value of fine cell (g0, g1, g2) after upsampling: f[k] * prod fs_d[|k_d|] for
+// a mode cell, 0 for padding.
 template <typename T, int DIM>
-__global__ void __launch_bounds__(kBlock)
-    upsample_kernel(int order_fft, const Cplx<T>* __restrict__ in, Dims3 st, Dims3 N,
-                    FsPtrs<T> fs, Cplx<T>* __restrict__ grid, Dims3 n) {
-  const IntType rows = (DIM >= 2 ? n.v[1] : 1) * (DIM >= 3 ? n.v[2] : 1);
-  for (IntType row = blockIdx.y; row < rows; row += gridDim.y) {
-    T frow = T(1);
-    bool live = true;
-    IntType off = 0;
-    if constexpr (DIM >= 2) {
-      const IntType g1 = row % n.v[1];
-      IntType k1;
-      live = cell_mode(g1, n.v[1], N.v[1], k1);
-      if (live) {
-        frow *= fs.p[1][k1 < 0 ? -k1 : k1];
-        off += mode_index(k1, N.v[1], order_fft) * st.v[1];
-      }
-      if constexpr (DIM >= 3) {
-        const IntType g2 = row / n.v[1];
-        IntType k2;
-        const bool l2 = cell_mode(g2, n.v[2], N.v[2], k2);
-        live = live && l2;
-        if (live) {
-          frow *= fs.p[2][k2 < 0 ? -k2 : k2];
-          off += mode_index(k2, N.v[2], order_fft) * st.v[2];
-        }
-      }
+__device__ inline Cplx<T> upsample_cell(int order_fft, const Cplx<T>* __restrict__ in,
+                                        const SampleGeom& g, const FsPtrs<T>& fs, int g0, int g1,
+                                        int g2) {
+  int k0, k1 = 0, k2 = 0;
+  if (!cell_mode32(g0, g.n[0], g.N[0], k0)) return Cplx<T>{T(0), T(0)};
+  if (DIM >= 2 && !cell_mode32(g1, g.n[1], g.N[1], k1)) return Cplx<T>{T(0), T(0)};
+  if (DIM >= 3 && !cell_mode32(g2, g.n[2], g.N[2], k2)) return Cplx<T>{T(0), T(0)};
+  long long off = static_cast<long long>(mode_index32(k0, g.N[0], order_fft)) * g.st[0];
+  T f = fs.p[0][k0 < 0 ? -k0 : k0];
+  if (DIM >= 2) {
+    off += static_cast<long long>(mode_index32(k1, g.N[1], order_fft)) * g.st[1];
+    f *= fs.p[1][k1 < 0 ? -k1 : k1];
+  }
+  if (DIM >= 3) {
+    off += static_cast<long long>(mode_index32(k2, g.N[2], order_fft)) * g.st[2];
+    f *= fs.p[2][k2 < 0 ? -k2 : k2];
+  }
+  const Cplx<T> a = in[off];
+  return Cplx<T>{a.x * f, a.y * f};
+}
+
+__device__ inline void store_pair(Cplx<float>* p, const Cplx<float>& a, const Cplx<float>& b) {
+  *reinterpret_cast<float4*>(p) = make_float4(a.x, a.y, b.x, b.y);
+}
+__device__ inline void store_pair(Cplx<double>* p, const Cplx<double>& a, const Cplx<double>& b) {
+  p[0] = a;
+  p[1] = b;
+}
+
+// Writes every cell of one fine grid (threads tid, tid + nthr, ...). With an
+// even x extent two neighbouring cells go out in one 16-byte (fp32) store.
+template <typename T, int DIM>
+__device__ inline void upsample_grid(int order_fft, const Cplx<T>* __restrict__ in,
+                                     const SampleGeom& g, const FsPtrs<T>& fs,
+                                     Cplx<T>* __restrict__ grid, std::size_t tid,
+                                     std::size_t nthr) {
+  const std::size_t n0 = static_cast<std::size_t>(g.n[0]);
+  const std::size_t cells = n0 * g.n[1] * g.n[2];
+  if ((n0 & 1) == 0) {
+    for (std::size_t q = tid; 2 * q < cells; q += nthr) {
+      const std::size_t idx = 2 * q;
+      const std::size_t row = idx / n0;
+      const int g0 = static_cast<int>(idx - row * n0);
+      const int g1 = DIM >= 2 ? static_cast<int>(row % g.n[1]) : 0;
+      const int g2 = DIM >= 3 ? static_cast<int>(row / g.n[1]) : 0;
+      store_pair(grid + idx, upsample_cell<T, DIM>(order_fft, in, g, fs, g0, g1, g2),
+                 upsample_cell<T, DIM>(order_fft, in, g, fs, g0 + 1, g1, g2));
     }
-    Cplx<T>* grow = grid + row * n.v[0];
-    for (IntType g0 = threadIdx.x + static_cast<IntType>(blockIdx.x) * kBlock; g0 < n.v[0];
-         g0 += static_cast<IntType>(gridDim.x) * kBlock) {
-      Cplx<T> v{T(0), T(0)};
-      IntType k0;
-      if (live && cell_mode(g0, n.v[0], N.v[0], k0)) {
-        const Cplx<T> a = in[off + mode_index(k0, N.v[0], order_fft) * st.v[0]];
-        const T f = frow * fs.p[0][k0 < 0 ? -k0 : k0];
-        v = Cplx<T>{a.x * f, a.y * f};
-      }
-      grow[g0] = v;
+  } else {
+    for (std::size_t idx = tid; idx < cells; idx += nthr) {
+      const std::size_t row = idx / n0;
+      const int g0 = static_cast<int>(idx - row * n0);
+      const int g1 = DIM >= 2 ? static_cast<int>(row % g.n[1]) : 0;
+      const int g2 = DIM >= 3 ? static_cast<int>(row / g.n[1]) : 0;
+      grid[idx] = upsample_cell<T, DIM>(order_fft, in, g, fs, g0, g1, g2);
     }
+  }
+}
+
+// Type-1 deconvolution of every mode of one transform: out[i] (strided) =
+// grid[k mod n] * prod fs_d[|k_d|].
+template <typename T, int DIM>
+__device__ inline void downsample_grid(int order_fft, const Cplx<T>* __restrict__ grid,
+                                       const SampleGeom& g, const FsPtrs<T>& fs,
+                                       Cplx<T>* __restrict__ out, std::size_t tid,
+                                       std::size_t nthr) {
+  const std::size_t N0 = static_cast<std::size_t>(g.N[0]);
+  const std::size_t modes = N0 * g.N[1] * g.N[2];
+  for (std::size_t idx = tid; idx < modes; idx += nthr) {
+    const std::size_t row = idx / N0;
+    const int i[3] = {static_cast<int>(idx - row * N0),
+                      DIM >= 2 ? static_cast<int>(row % g.N[1]) : 0,
+                      DIM >= 3 ? static_cast<int>(row / g.N[1]) : 0};
+    long long goff = 0, ooff = 0, gmul = 1;
+    T f = T(1);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int N = g.N[d];
+      const int k = order_fft ? (i[d] < N - N / 2 ? i[d] : i[d] - N) : i[d] - N / 2;
+      f *= fs.p[d][k < 0 ? -k : k];
+      goff += static_cast<long long>(k < 0 ? k + g.n[d] : k) * gmul;
+      gmul *= g.n[d];
+      ooff += static_cast<long long>(i[d]) * g.st[d];
+    }
+    const Cplx<T> a = grid[goff];
+    out[ooff] = Cplx<T>{a.x * f, a.y * f};
   }
 }
 
 template <typename T, int DIM>
 __global__ void __launch_bounds__(kBlock)
-    downsample_kernel(int order_fft, const Cplx<T>* __restrict__ grid, Dims3 n, FsPtrs<T> fs,
-                      Cplx<T>* __restrict__ out, Dims3 st, Dims3 N) {
-  const IntType rows = (DIM >= 2 ? N.v[1] : 1) * (DIM >= 3 ? N.v[2] : 1);
-  for (IntType row = blockIdx.y; row < rows; row += gridDim.y) {
-    T frow = T(1);
-    IntType goff = 0, ooff = 0;
-    if constexpr (DIM >= 2) {
-      const IntType i1 = row % N.v[1];
-      const IntType k1 = order_fft ? (i1 < N.v[1] - N.v[1] / 2 ? i1 : i1 - N.v[1]) : i1 - N.v[1] / 2;
-      frow *= fs.p[1][k1 < 0 ? -k1 : k1];
-      goff += (k1 < 0 ? k1 + n.v[1] : k1);
-      ooff += i1 * st.v[1];
-      if constexpr (DIM >= 3) {
-        const IntType i2 = row / N.v[1];
-        const IntType k2 =
-            order_fft ? (i2 < N.v[2] - N.v[2] / 2 ? i2 : i2 - N.v[2]) : i2 - N.v[2] / 2;
-        frow *= fs.p[2][k2 < 0 ? -k2 : k2];
-        goff += (k2 < 0 ? k2 + n.v[2] : k2) * n.v[1];
-        ooff += i2 * st.v[2];
-      }
-    }
-    const Cplx<T>* grow = grid + goff * n.v[0];
-    for (IntType i0 = threadIdx.x + static_cast<IntType>(blockIdx.x) * kBlock; i0 < N.v[0];
-         i0 += static_cast<IntType>(gridDim.x) * kBlock) {
-      const IntType k0 =
-          order_fft ? (i0 < N.v[0] - N.v[0] / 2 ? i0 : i0 - N.v[0]) : i0 - N.v[0] / 2;
-      const Cplx<T> a = grow[k0 < 0 ? k0 + n.v[0] : k0];
-      const T f = frow * fs.p[0][k0 < 0 ? -k0 : k0];
-      out[ooff + i0 * st.v[0]] = Cplx<T>{a.x * f, a.y * f};
-    }
-  }
-}
-
-// Batched variants: blockIdx.z selects one of `count` transforms of equal
-// mode and fine-grid shape whose grids are contiguous (stride `gstride`).
-template <typename T, int DIM>
-__global__ void __launch_bounds__(kBlock)
-    upsample_batch_kernel(int order_fft, const Cplx<T>* const* __restrict__ ins, Dims3 st,
-                          Dims3 N, FsPtrs<T> fs, Cplx<T>* __restrict__ grids, std::size_t gstride,
-                          Dims3 n) {
-  const Cplx<T>* __restrict__ in = ins[blockIdx.z];
-  Cplx<T>* __restrict__ grid = grids + gstride * blockIdx.z;
-  const IntType rows = (DIM >= 2 ? n.v[1] : 1) * (DIM >= 3 ? n.v[2] : 1);
-  for (IntType row = blockIdx.y; row < rows; row += gridDim.y) {
-    T frow = T(1);
-    bool live = true;
-    IntType off = 0;
-    if constexpr (DIM >= 2) {
-      const IntType g1 = row % n.v[1];
-      IntType k1;
-      live = cell_mode(g1, n.v[1], N.v[1], k1);
-      if (live) {
-        frow *= fs.p[1][k1 < 0 ? -k1 : k1];
-        off += mode_index(k1, N.v[1], order_fft) * st.v[1];
-      }
-      if constexpr (DIM >= 3) {
-        const IntType g2 = row / n.v[1];
-        IntType k2;
-        const bool l2 = cell_mode(g2, n.v[2], N.v[2], k2);
-        live = live && l2;
-        if (live) {
-          frow *= fs.p[2][k2 < 0 ? -k2 : k2];
-          off += mode_index(k2, N.v[2], order_fft) * st.v[2];
-        }
-      }
-    }
-    Cplx<T>* grow = grid + row * n.v[0];
-    for (IntType g0 = threadIdx.x + static_cast<IntType>(blockIdx.x) * kBlock; g0 < n.v[0];
-         g0 += static_cast<IntType>(gridDim.x) * kBlock) {
-      Cplx<T> v{T(0), T(0)};
-      IntType k0;
-      if (live && cell_mode(g0, n.v[0], N.v[0], k0)) {
-        const Cplx<T> a = in[off + mode_index(k0, N.v[0], order_fft) * st.v[0]];
-        const T f = frow * fs.p[0][k0 < 0 ? -k0 : k0];
-        v = Cplx<T>{a.x * f, a.y * f};
-      }
-      grow[g0] = v;
-    }
-  }
+    upsample_kernel(int order_fft, const Cplx<T>* __restrict__ in, SampleGeom g, FsPtrs<T> fs,
+                    Cplx<T>* __restrict__ grid) {
+  upsample_grid<T, DIM>(order_fft, in, g, fs, grid,
+                        threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock,
+                        static_cast<std::size_t>(gridDim.x) * kBlock);
 }
 
 template <typename T, int DIM>
 __global__ void __launch_bounds__(kBlock)
-    downsample_batch_kernel(int order_fft, const Cplx<T>* __restrict__ grids, std::size_t gstride,
-                            Dims3 n, FsPtrs<T> fs, Cplx<T>* const* __restrict__ outs, Dims3 st,
-                            Dims3 N) {
-  const Cplx<T>* __restrict__ grid = grids + gstride * blockIdx.z;
-  Cplx<T>* __restrict__ out = outs[blockIdx.z];
-  const IntType rows = (DIM >= 2 ? N.v[1] : 1) * (DIM >= 3 ? N.v[2] : 1);
-  for (IntType row = blockIdx.y; row < rows; row += gridDim.y) {
-    T frow = T(1);
-    IntType goff = 0, ooff = 0;
-    if constexpr (DIM >= 2) {
-      const IntType i1 = row % N.v[1];
-      const IntType k1 = order_fft ? (i1 < N.v[1] - N.v[1] / 2 ? i1 : i1 - N.v[1]) : i1 - N.v[1] / 2;
-      frow *= fs.p[1][k1 < 0 ? -k1 : k1];
-      goff += (k1 < 0 ? k1 + n.v[1] : k1);
-      ooff += i1 * st.v[1];
-      if constexpr (DIM >= 3) {
-        const IntType i2 = row / N.v[1];
-        const IntType k2 =
-            order_fft ? (i2 < N.v[2] - N.v[2] / 2 ? i2 : i2 - N.v[2]) : i2 - N.v[2] / 2;
-        frow *= fs.p[2][k2 < 0 ? -k2 : k2];
-        goff += (k2 < 0 ? k2 + n.v[2] : k2) * n.v[1];
-        ooff += i2 * st.v[2];
-      }
-    }
-    const Cplx<T>* grow = grid + goff * n.v[0];
-    for (IntType i0 = threadIdx.x + static_cast<IntType>(blockIdx.x) * kBlock; i0 < N.v[0];
-         i0 += static_cast<IntType>(gridDim.x) * kBlock) {
-      const IntType k0 =
-          order_fft ? (i0 < N.v[0] - N.v[0] / 2 ? i0 : i0 - N.v[0]) : i0 - N.v[0] / 2;
-      const Cplx<T> a = grow[k0 < 0 ? k0 + n.v[0] : k0];
-      const T f = frow * fs.p[0][k0 < 0 ? -k0 : k0];
-      out[ooff + i0 * st.v[0]] = Cplx<T>{a.x * f, a.y * f};
-    }
-  }
+    downsample_kernel(int order_fft, const Cplx<T>* __restrict__ grid, SampleGeom g,
+                      FsPtrs<T> fs, Cplx<T>* __restrict__ out) {
+  downsample_grid<T, DIM>(order_fft, grid, g, fs, out,
+                          threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock,
+                          static_cast<std::size_t>(gridDim.x) * kBlock);
+}
+
+// Batched variants: blockIdx.y selects one of the transforms of equal mode and
+// fine-grid shape whose grids are contiguous with stride `gstride` cells.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kBlock)
+    upsample_batch_kernel(int order_fft, const Cplx<T>* const* __restrict__ ins, SampleGeom g,
+                          FsPtrs<T> fs, Cplx<T>* __restrict__ grids, std::size_t gstride) {
+  upsample_grid<T, DIM>(order_fft, ins[blockIdx.y], g, fs, grids + gstride * blockIdx.y,
+                        threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock,
+                        static_cast<std::size_t>(gridDim.x) * kBlock);
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kBlock)
+    downsample_batch_kernel(int order_fft, const Cplx<T>* __restrict__ grids,
+                            std::size_t gstride, SampleGeom g, FsPtrs<T> fs,
+                            Cplx<T>* const* __restrict__ outs) {
+  downsample_grid<T, DIM>(order_fft, grids + gstride * blockIdx.y, g, fs, outs[blockIdx.y],
+                          threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock,
+                          static_cast<std::size_t>(gridDim.x) * kBlock);
 }
 
 struct Vec3d {
@@ -319,22 +289,35 @@ void launch_fseries(T* fs, IntType n, const QuadRule& rule, hipStream_t stream) 
 }
 
 template <typename T, int DIM>
+SampleGeom sample_geom(const IntType* n, const IntType* modes, const IntType* strides) {
+  SampleGeom g{{1, 1, 1}, {1, 1, 1}, {0, 0, 0}};
+  for (int d = 0; d < DIM; ++d) {
+    g.n[d] = static_cast<int>(n[d]);
+    g.N[d] = static_cast<int>(modes[d]);
+    g.st[d] = static_cast<long long>(strides[d]);
+  }
+  return g;
+}
+
+template <typename T, int DIM>
+FsPtrs<T> fs_ptrs(T* const* fs) {
+  FsPtrs<T> f{{nullptr, nullptr, nullptr}};
+  for (int d = 0; d < DIM; ++d) f.p[d] = fs[d];
+  return f;
+}
+
+inline std::size_t up_work(const SampleGeom& g) {
+  const std::size_t cells = static_cast<std::size_t>(g.n[0]) * g.n[1] * g.n[2];
+  return (g.n[0] & 1) == 0 ? cells / 2 : cells;
+}
+
+template <typename T, int DIM>
 void launch_upsample(int order_fft, const Cplx<T>* in, const IntType* in_strides,
                      const IntType* modes, T* const* fs, Cplx<T>* grid, const IntType* n,
                      hipStream_t stream) {
-  Dims3 st{{0, 0, 0}}, NN{{1, 1, 1}}, nn{{1, 1, 1}};
-  FsPtrs<T> f{{nullptr, nullptr, nullptr}};
-  for (int d = 0; d < DIM; ++d) {
-    st.v[d] = in_strides[d];
-    NN.v[d] = modes[d];
-    nn.v[d] = n[d];
-    f.p[d] = fs[d];
-  }
-  const IntType rows = nn.v[1] * nn.v[2];
-  const unsigned gx = static_cast<unsigned>((nn.v[0] + kBlock - 1) / kBlock);
-  const unsigned gy = static_cast<unsigned>(rows < 65535 ? rows : 65535);
-  hipLaunchKernelGGL((upsample_kernel<T, DIM>), dim3(gx, gy), dim3(kBlock), 0, stream, order_fft,
-                     in, st, NN, f, grid, nn);
+  const SampleGeom g = sample_geom<T, DIM>(n, modes, in_strides);
+  hipLaunchKernelGGL((upsample_kernel<T, DIM>), dim3(blocks_for(up_work(g))), dim3(kBlock), 0,
+                     stream, order_fft, in, g, fs_ptrs<T, DIM>(fs), grid);
   check_launch("upsample_kernel");
 }
 
@@ -342,19 +325,10 @@ template <typename T, int DIM>
 void launch_downsample(int order_fft, const Cplx<T>* grid, const IntType* n, T* const* fs,
                        Cplx<T>* out, const IntType* out_strides, const IntType* modes,
                        hipStream_t stream) {
-  Dims3 st{{0, 0, 0}}, NN{{1, 1, 1}}, nn{{1, 1, 1}};
-  FsPtrs<T> f{{nullptr, nullptr, nullptr}};
-  for (int d = 0; d < DIM; ++d) {
-    st.v[d] = out_strides[d];
-    NN.v[d] = modes[d];
-    nn.v[d] = n[d];
-    f.p[d] = fs[d];
-  }
-  const IntType rows = NN.v[1] * NN.v[2];
-  const unsigned gx = static_cast<unsigned>((NN.v[0] + kBlock - 1) / kBlock);
-  const unsigned gy = static_cast<unsigned>(rows < 65535 ? rows : 65535);
-  hipLaunchKernelGGL((downsample_kernel<T, DIM>), dim3(gx, gy), dim3(kBlock), 0, stream,
-                     order_fft, grid, nn, f, out, st, NN);
+  const SampleGeom g = sample_geom<T, DIM>(n, modes, out_strides);
+  const std::size_t work = static_cast<std::size_t>(g.N[0]) * g.N[1] * g.N[2];
+  hipLaunchKernelGGL((downsample_kernel<T, DIM>), dim3(blocks_for(work)), dim3(kBlock), 0,
+                     stream, order_fft, grid, g, fs_ptrs<T, DIM>(fs), out);
   check_launch("downsample_kernel");
 }
 
@@ -363,19 +337,12 @@ void launch_upsample_batch(int order_fft, const Cplx<T>* const* ins, const IntTy
                            const IntType* modes, T* const* fs, Cplx<T>* grids,
                            std::size_t gstride, const IntType* n, int count, hipStream_t stream) {
   if (count <= 0) return;
-  Dims3 st{{0, 0, 0}}, NN{{1, 1, 1}}, nn{{1, 1, 1}};
-  FsPtrs<T> f{{nullptr, nullptr, nullptr}};
-  for (int d = 0; d < DIM; ++d) {
-    st.v[d] = in_strides[d];
-    NN.v[d] = modes[d];
-    nn.v[d] = n[d];
-    f.p[d] = fs[d];
-  }
-  const IntType rows = nn.v[1] * nn.v[2];
-  const unsigned gx = static_cast<unsigned>((nn.v[0] + kBlock - 1) / kBlock);
-  const unsigned gy = static_cast<unsigned>(rows < 1024 ? rows : 1024);
-  hipLaunchKernelGGL((upsample_batch_kernel<T, DIM>), dim3(gx, gy, static_cast<unsigned>(count)),
-                     dim3(kBlock), 0, stream, order_fft, ins, st, NN, f, grids, gstride, nn);
+  if (count > 65535) throw neonufft::InputError("Neonufft:: too many transforms of one shape");
+  const SampleGeom g = sample_geom<T, DIM>(n, modes, in_strides);
+  hipLaunchKernelGGL((upsample_batch_kernel<T, DIM>),
+                     dim3(blocks_for(up_work(g), 2048u), static_cast<unsigned>(count)),
+                     dim3(kBlock), 0, stream, order_fft, ins, g, fs_ptrs<T, DIM>(fs), grids,
+                     gstride);
   check_launch("upsample_batch_kernel");
 }
 
@@ -385,19 +352,12 @@ void launch_downsample_batch(int order_fft, const Cplx<T>* grids, std::size_t gs
                              const IntType* out_strides, const IntType* modes, int count,
                              hipStream_t stream) {
   if (count <= 0) return;
-  Dims3 st{{0, 0, 0}}, NN{{1, 1, 1}}, nn{{1, 1, 1}};
-  FsPtrs<T> f{{nullptr, nullptr, nullptr}};
-  for (int d = 0; d < DIM; ++d) {
-    st.v[d] = out_strides[d];
-    NN.v[d] = modes[d];
-    nn.v[d] = n[d];
-    f.p[d] = fs[d];
-  }
-  const IntType rows = NN.v[1] * NN.v[2];
-  const unsigned gx = static_cast<unsigned>((NN.v[0] + kBlock - 1) / kBlock);
-  const unsigned gy = static_cast<unsigned>(rows < 1024 ? rows : 1024);
-  hipLaunchKernelGGL((downsample_batch_kernel<T, DIM>), dim3(gx, gy, static_cast<unsigned>(count)),
-                     dim3(kBlock), 0, stream, order_fft, grids, gstride, nn, f, outs, st, NN);
+  if (count > 65535) throw neonufft::InputError("Neonufft:: too many transforms of one shape");
+  const SampleGeom g = sample_geom<T, DIM>(n, modes, out_strides);
+  const std::size_t work = static_cast<std::size_t>(g.N[0]) * g.N[1] * g.N[2];
+  hipLaunchKernelGGL((downsample_batch_kernel<T, DIM>),
+                     dim3(blocks_for(work, 2048u), static_cast<unsigned>(count)), dim3(kBlock),
+                     0, stream, order_fft, grids, gstride, g, fs_ptrs<T, DIM>(fs), outs);
   check_launch("downsample_batch_kernel");
 }
 

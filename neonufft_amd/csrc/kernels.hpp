@@ -61,10 +61,62 @@ template <typename T>
 void launch_minmax(const T* x, std::size_t m, double* partial, hipStream_t stream);
 
 // ---- spread_interp.hip: ES spreading / interpolation on binned points -----
+// Cells of one subgrid tile (bin + ns halo), the pitch of every stored tile.
+__host__ __device__ inline std::size_t tile_cells(const BinGeom& g) {
+  return static_cast<std::size_t>(g.tile[0]) * g.tile[1] * g.tile[2];
+}
+
+// Spreading reads strength in[perm[p]] for sorted point p (in[p] when perm is
+// null, i.e. strengths already in bin order) and writes the whole tile of
+// subproblem s (tile_cells(g) cells, x
+// fastest) to tiles + s * tile_cells(g); launch_tile_reduce then forms the
+// fine grid from the tiles.
 template <typename T, int DIM>
 void launch_spread(int ns, const PointRec<T, DIM>* pts, const uint32_t* perm, const Subprob* subs,
-                   std::size_t nsub, const Cplx<T>* in, const Cplx<T>* pre_sorted, Cplx<T>* grid,
+                   std::size_t nsub, const Cplx<T>* in, const Cplx<T>* pre_sorted, Cplx<T>* tiles,
                    const BinGeom& g, const EsParams& es, hipStream_t stream);
+
+// Fine-grid cell -> covering tiles, one table per dimension: entries
+// [start[d][c], start[d][c + 1]) of entry[d] hold (bin coordinate, position
+// inside that bin's tile) for cell c of dimension d (built on the host).
+struct CoverTable {
+  const int* start[3];
+  const int2* entry[3];
+};
+
+// Everything the tile reduction needs for one fine grid.
+template <typename T, int DIM>
+struct ReduceJob {
+  const Cplx<T>* tiles;       // tile of (local) subproblem s at tiles + s * tile_cells(g)
+  const uint32_t* sub_start;  // nbins + 1: subproblems of bin b are [sub_start[b], sub_start[b+1])
+  Cplx<T>* grid;
+  BinGeom g;
+  CoverTable cover;
+};
+
+// True when, in every dimension, a cell is covered only by the tiles of its
+// own bin and its two neighbours: bin >= ns, at least two bins, and the last
+// bin full or at least ns wide. Such geometries use the bin-owned reduction.
+__host__ __device__ inline bool bins_regular(int dim, const BinGeom& g, int ns) {
+  for (int d = 0; d < dim; ++d) {
+    const int rem = g.n[d] % g.bin[d];
+    if (g.nbins[d] < 2 || g.bin[d] < ns || (rem != 0 && rem < ns)) return false;
+  }
+  return true;
+}
+
+// grid[c] (+)= sum over the tiles covering cell c; every cell is written, so
+// no zero fill is needed before a non-accumulating spread.
+template <typename T, int DIM>
+void launch_tile_reduce(const ReduceJob<T, DIM>& job, int ns, bool accumulate,
+                        hipStream_t stream);
+// One reduction per job (device table of `count` jobs) in a single launch.
+// regular: every job's geometry satisfies bins_regular; max_bins / max_n0 /
+// max_rows: the largest bin count, x extent and row count among the grids.
+template <typename T, int DIM>
+void launch_tile_reduce_batch(const ReduceJob<T, DIM>* jobs, int count, int ns, bool regular,
+                              std::size_t max_bins, int max_n0, int max_rows, bool accumulate,
+                              hipStream_t stream);
 
 template <typename T, int DIM>
 void launch_interp(int ns, const PointRec<T, DIM>* pts, const uint32_t* perm, const Subprob* subs,
@@ -76,10 +128,12 @@ void launch_interp(int ns, const PointRec<T, DIM>* pts, const uint32_t* perm, co
 template <typename T, int DIM>
 struct SpreadJob {
   const PointRec<T, DIM>* pts;
-  const uint32_t* perm;
+  const uint32_t* perm;  // null: `in` is already in bin order
   const Cplx<T>* in;
   const Cplx<T>* pre;  // bin-order prephase or null
-  Cplx<T>* grid;
+  Cplx<T>* tiles;      // this transform's tiles, indexed by subproblem - sub_off
+  uint32_t sub_off;    // first subproblem of this transform in the shared table
+  uint32_t pad;        // keeps the struct free of indeterminate padding bytes
   BinGeom g;
 };
 

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include <neonufft_amd.h>
+#include "cover.hpp"
 #include "fft.hpp"
 #include "params.hpp"
 #include "point_set.hpp"
@@ -56,6 +57,7 @@ public:
       launch_fseries<T>(fs_[d].data(), n_[d], ks_.quad, stream_);
     }
     geom_ = choose_bins(DIM, n_, ks_.ns, sizeof(Cplx<T>), tune_.bin);
+    cover_.build(DIM, geom_, ks_.ns, stream_, alloc_);
     set_points(num_nu, loc);
   }
 
@@ -66,15 +68,19 @@ public:
     FoldParams fp{{0, 0, 0}, {1, 1, 1}};
     pts_.build(l, static_cast<std::size_t>(num_nu), fp, geom_, ks_.ns, tune_.max_points, stream_,
                alloc_);
+    tiles_.ensure(std::max<std::size_t>(pts_.nsub(), 1) * tile_cells(geom_), alloc_);
     check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   }
 
-  // Stage 1 of type 1: zero the fine grid (unless accumulating), then spread.
+  // Stage 1 of type 1: spread into per-subproblem tiles, then form the fine
+  // grid from them (overwriting it, or adding to it when accumulating).
   void spread(const Cplx<T>* in, bool accumulate = false) {
     if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
-    if (!accumulate) launch_fill_zero<T>(grid_.data(), grid_.size(), stream_);
     launch_spread<T, DIM>(ks_.ns, pts_.recs(), pts_.perm(), pts_.subs(), pts_.nsub(), in, nullptr,
-                          grid_.data(), geom_, ks_.es, stream_);
+                          tiles_.data(), geom_, ks_.es, stream_);
+    const ReduceJob<T, DIM> job{tiles_.data(), pts_.sub_starts(), grid_.data(), geom_,
+                                cover_.table()};
+    launch_tile_reduce<T, DIM>(job, ks_.ns, accumulate, stream_);
   }
 
   // Stage 3 of type 2: interpolate the current fine grid.
@@ -135,6 +141,8 @@ private:
   IntType n_[3];
   BinGeom geom_{};
   DeviceBuffer<Cplx<T>> grid_;
+  DeviceBuffer<Cplx<T>> tiles_;  // spreading tiles, tile_cells(geom_) per subproblem
+  CoverBuffers cover_;
   FftPlan fft_;
   DeviceBuffer<T> fs_[3];
   PointSet<T, DIM> pts_;
@@ -199,6 +207,7 @@ public:
     }
     in_pts_.build(l, static_cast<std::size_t>(num_in), fp, geom_in_, ks_.ns, tune_.max_points,
                   stream_, alloc_);
+    tiles_.ensure(std::max<std::size_t>(in_pts_.nsub(), 1) * tile_cells(geom_in_), alloc_);
     has_pre_ = false;
     for (int d = 0; d < DIM; ++d) has_pre_ = has_pre_ || gi_.output_offsets[d] != 0.0;
     if (has_pre_) {
@@ -240,8 +249,12 @@ public:
         src = in + b * bdist;
       }
       launch_spread<T, DIM>(ks_.ns, in_pts_.recs(), in_pts_.perm(), in_pts_.subs(), in_pts_.nsub(),
-                            src, has_pre_ ? pre_.data() : nullptr,
-                            spread_grid_.data() + b * spread_cells_, geom_in_, ks_.es, stream_);
+                            src, has_pre_ ? pre_.data() : nullptr, tiles_.data(), geom_in_,
+                            ks_.es, stream_);
+      const ReduceJob<T, DIM> job{tiles_.data(), in_pts_.sub_starts(),
+                                  spread_grid_.data() + b * spread_cells_, geom_in_,
+                                  cover_in_.table()};
+      launch_tile_reduce<T, DIM>(job, ks_.ns, true, stream_);
     }
   }
 
@@ -362,6 +375,7 @@ private:
     }
     geom_in_ = choose_bins(DIM, gi_.spread_grid_size, ks_.ns, sizeof(Cplx<T>), tune_.bin);
     geom_out_ = choose_bins(DIM, gi_.fft_grid_size, ks_.ns, sizeof(Cplx<T>), tune_.bin);
+    cover_in_.build(DIM, geom_in_, ks_.ns, stream_, alloc_);
     check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   }
 
@@ -377,6 +391,8 @@ private:
   BinGeom geom_in_{}, geom_out_{};
   DeviceBuffer<Cplx<T>> spread_grid_, fft_grid_, pre_, post_;
   DeviceBuffer<Cplx<T>> io_in_, io_out_;  // mixed precision: converted inputs / outputs
+  DeviceBuffer<Cplx<T>> tiles_;           // spreading tiles of the input points
+  CoverBuffers cover_in_;
   bool has_pre_ = false;
   FftPlan fft_;
   DeviceBuffer<T> fs_[3];

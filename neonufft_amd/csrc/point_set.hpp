@@ -65,6 +65,9 @@ public:
   const Subprob* subs() const { return subs_.data(); }
   const uint32_t* bin_counts() const { return count_.data(); }
   const uint32_t* bin_starts() const { return start_.data(); }
+  // Per bin, nbins + 1 entries: the subproblems of bin b are
+  // [sub_starts()[b], sub_starts()[b + 1]) in subs().
+  const uint32_t* sub_starts() const { return sub_start_.data(); }
   std::size_t nsub() const { return nsub_; }
   std::size_t size() const { return m_; }
   std::size_t nbins() const { return nbins_; }

@@ -8,9 +8,10 @@
 // postphase). The design is MI355X-specific:
 //  - one workgroup per subproblem (<= max_points points of one bin), with the
 //    bin's tile plus its ns-cell halo staged in LDS;
-//  - spreading accumulates with LDS atomics and flushes the tile to HBM once
-//    with coalesced float atomics; interpolation loads the tile once with
-//    coalesced reads and gathers from LDS;
+//  - spreading accumulates with LDS atomics, stores the finished tile with
+//    plain coalesced stores, and a tile reduction sums overlapping tiles into
+//    the grid (no global atomics, fixed summation order); interpolation
+//    loads the tile once with coalesced reads and gathers from LDS;
 //  - kernel weights come from per-piece Horner polynomials in registers (fp32)
 //    or the exact exp/sqrt formula (fp64);
 //  - subproblem -> workgroup mapping keeps neighbouring bins on one XCD.
@@ -81,12 +82,6 @@ __device__ inline int unpack_cell(uint32_t cell, int d) {
   }
 }
 
-__device__ inline void hbm_add(float* p, float v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void hbm_add(double* p, double v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Tile geometry of one subproblem.
 template <int DIM, int NS>
@@ -169,14 +164,14 @@ __device__ inline T block_sum(T v, T* scratch) {
 // One subproblem of spreading. `smem_raw` is the dynamic LDS tile.
 template <typename T, int DIM, int NS>
 __device__ __forceinline__ void spread_subproblem(const SpreadJob<T, DIM>& job,
-                                                  const Subprob& sp, const EsParams& es,
-                                                  unsigned char* smem_raw, T* wsum) {
+                                                  const Subprob& sp, uint32_t sidx,
+                                                  const EsParams& es, unsigned char* smem_raw,
+                                                  T* wsum) {
   using Acc = typename FixedAcc<T>::type;
   const PointRec<T, DIM>* __restrict__ pts = job.pts;
   const uint32_t* __restrict__ perm = job.perm;
   const Cplx<T>* __restrict__ in = job.in;
   const Cplx<T>* __restrict__ pre = job.pre;
-  Cplx<T>* __restrict__ grid = job.grid;
   const BinGeom& g = job.g;
   const TileInfo<DIM, NS> ti(sp.bin, g);
   const int tile_cells = g.tile[0] * (DIM >= 2 ? g.tile[1] : 1) * (DIM >= 3 ? g.tile[2] : 1);
@@ -197,7 +192,7 @@ __device__ __forceinline__ void spread_subproblem(const SpreadJob<T, DIM>& job,
     cv[k] = Cplx<T>{T(0), T(0)};
     if (p < end) {
       rec[k] = pts[p];
-      Cplx<T> c = in[perm[p]];
+      Cplx<T> c = perm ? in[perm[p]] : in[p];
       if (pre) {
         const Cplx<T> q = pre[p];
         c = Cplx<T>{c.x * q.x - c.y * q.y, c.x * q.y + c.y * q.x};
@@ -252,29 +247,14 @@ __device__ __forceinline__ void spread_subproblem(const SpreadJob<T, DIM>& job,
   }
   __syncthreads();
 
-  // Flush the tile: rows along x are added to the grid with coalesced float
-  // atomics (cheap at this access pattern).
+  // Store the whole tile (converted back from fixed point) to this
+  // subproblem's slot with coalesced plain stores; the tile reduction adds
+  // overlapping tiles into the grid without atomics.
   const T inv = total > T(0) ? total / one : T(0);
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int rows = ti.ext[1] * ti.ext[2];
-  for (int r = wave; r < rows; r += kThreads / 64) {
-    const int i1 = r % ti.ext[1];
-    const int i2 = r / ti.ext[1];
-    const int gy = DIM >= 2 ? wrap_once(ti.org[1] + i1, g.n[1]) : 0;
-    const int gz = DIM >= 3 ? wrap_once(ti.org[2] + i2, g.n[2]) : 0;
-    const std::size_t grow =
-        (static_cast<std::size_t>(gz) * static_cast<std::size_t>(g.n[1]) + gy) * g.n[0];
-    const int trow = (i2 * ti.t1 + i1) * ti.t0;
-    for (int i0 = lane; i0 < ti.ext[0]; i0 += 64) {
-      const Acc qr = tre[trow + i0];
-      const Acc qi = tim[trow + i0];
-      if (qr != Acc(0) || qi != Acc(0)) {
-        T* dst = reinterpret_cast<T*>(grid + grow + wrap_once(ti.org[0] + i0, g.n[0]));
-        hbm_add(dst, static_cast<T>(qr) * inv);
-        hbm_add(dst + 1, static_cast<T>(qi) * inv);
-      }
-    }
+  Cplx<T>* __restrict__ dst =
+      job.tiles + static_cast<std::size_t>(sidx - job.sub_off) * static_cast<std::size_t>(tile_cells);
+  for (int i = threadIdx.x; i < tile_cells; i += kThreads) {
+    dst[i] = Cplx<T>{static_cast<T>(tre[i]) * inv, static_cast<T>(tim[i]) * inv};
   }
 }
 
@@ -284,8 +264,8 @@ __global__ void __launch_bounds__(kThreads)
                   EsParams es) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ T wsum[kThreads / 64];
-  const Subprob sp = subs[xcd_remap(blockIdx.x, nsub)];
-  spread_subproblem<T, DIM, NS>(job, sp, es, smem_raw, wsum);
+  const uint32_t sidx = xcd_remap(blockIdx.x, nsub);
+  spread_subproblem<T, DIM, NS>(job, subs[sidx], sidx, es, smem_raw, wsum);
 }
 
 // Many independent transforms in one launch: each subproblem names its job.
@@ -295,9 +275,10 @@ __global__ void __launch_bounds__(kThreads)
                         const Subprob* __restrict__ subs, uint32_t nsub, EsParams es) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ T wsum[kThreads / 64];
-  const Subprob sp = subs[xcd_remap(blockIdx.x, nsub)];
+  const uint32_t sidx = xcd_remap(blockIdx.x, nsub);
+  const Subprob sp = subs[sidx];
   const SpreadJob<T, DIM> job = jobs[sp.job];
-  spread_subproblem<T, DIM, NS>(job, sp, es, smem_raw, wsum);
+  spread_subproblem<T, DIM, NS>(job, sp, sidx, es, smem_raw, wsum);
 }
 
 // ---------------------------------------------------------------------------
@@ -444,6 +425,336 @@ void interp_batch_ns(const InterpJob<T, DIM>* jobs, const Subprob* subs, std::si
   check_launch("interp_batch_kernel");
 }
 
+// ---------------------------------------------------------------------------
+// Tile reduction: fine-grid cell c receives the sum of every stored tile that
+// covers it (each subproblem of each covering bin), read at the cell's
+// position inside that tile. Consecutive threads take consecutive x cells, so
+// the tile reads and the grid write are coalesced; no atomics, and the
+// summation order is fixed.
+// ---------------------------------------------------------------------------
+constexpr int kReduceThreads = 256;
+
+// Adds tile values of every subproblem of `bin` at position `pos` (general
+// path: any number of covering tiles per dimension).
+template <typename T, int DIM>
+__device__ inline void reduce_cell_loops(const ReduceJob<T, DIM>& job, int x0, int x1, int y0,
+                                         int y1, int z0, int z1, T& ar, T& ai) {
+  const BinGeom& g = job.g;
+  const std::size_t tc = tile_cells(g);
+  for (int iz = z0; iz < z1; ++iz) {
+    int bz = 0, lz = 0;
+    if constexpr (DIM >= 3) {
+      const int2 e = job.cover.entry[2][iz];
+      bz = e.x;
+      lz = e.y;
+    }
+    for (int iy = y0; iy < y1; ++iy) {
+      int by = 0, ly = 0;
+      if constexpr (DIM >= 2) {
+        const int2 e = job.cover.entry[1][iy];
+        by = e.x;
+        ly = e.y;
+      }
+      const int brow = g.nbins[0] * (by + g.nbins[1] * bz);
+      const std::size_t lrow = static_cast<std::size_t>(g.tile[0]) * (ly + g.tile[1] * lz);
+      for (int ix = x0; ix < x1; ++ix) {
+        const int2 e = job.cover.entry[0][ix];
+        const int bin = brow + e.x;
+        const uint32_t s0 = job.sub_start[bin], s1 = job.sub_start[bin + 1];
+        const Cplx<T>* t = job.tiles + lrow + e.y;
+        for (uint32_t s = s0; s < s1; ++s) {
+          const Cplx<T> v = t[static_cast<std::size_t>(s) * tc];
+          ar += v.x;
+          ai += v.y;
+        }
+      }
+    }
+  }
+}
+
+// Sums the tiles covering the x cells of one grid row (y, z fixed: the y / z
+// cover lists are the same for every thread of the block). The common case,
+// at most two covering tiles per dimension, evaluates all (<= 8) tile
+// combinations unrolled so their loads are in flight together; other cells
+// take the general loops.
+template <typename T, int DIM>
+__device__ inline void reduce_row(const ReduceJob<T, DIM>& job, int c1, int c2, int xbegin,
+                                  int xstride, bool accumulate) {
+  const BinGeom& g = job.g;
+  const std::size_t tc = tile_cells(g);
+  int y0 = 0, y1 = 1, z0 = 0, z1 = 1;
+  if constexpr (DIM >= 2) {
+    y0 = job.cover.start[1][c1];
+    y1 = job.cover.start[1][c1 + 1];
+  }
+  if constexpr (DIM >= 3) {
+    z0 = job.cover.start[2][c2];
+    z1 = job.cover.start[2][c2 + 1];
+  }
+  constexpr int NY = DIM >= 2 ? 2 : 1;
+  constexpr int NZ = DIM >= 3 ? 2 : 1;
+  const bool yz_fast = (y1 - y0) <= NY && (z1 - z0) <= NZ;
+  // row part of every combination: bin row offset and tile row offset
+  int brow[NY * NZ];
+  std::size_t lrow[NY * NZ];
+  bool vrow[NY * NZ];
+#pragma unroll
+  for (int jz = 0; jz < NZ; ++jz) {
+#pragma unroll
+    for (int jy = 0; jy < NY; ++jy) {
+      const int k = jy + NY * jz;
+      vrow[k] = yz_fast && (y0 + jy < y1) && (z0 + jz < z1);
+      int by = 0, ly = 0, bz = 0, lz = 0;
+      if (vrow[k]) {
+        if constexpr (DIM >= 2) {
+          const int2 e = job.cover.entry[1][y0 + jy];
+          by = e.x;
+          ly = e.y;
+        }
+        if constexpr (DIM >= 3) {
+          const int2 e = job.cover.entry[2][z0 + jz];
+          bz = e.x;
+          lz = e.y;
+        }
+      }
+      brow[k] = g.nbins[0] * (by + g.nbins[1] * bz);
+      lrow[k] = static_cast<std::size_t>(g.tile[0]) * (ly + g.tile[1] * lz);
+    }
+  }
+  Cplx<T>* __restrict__ grow =
+      job.grid + static_cast<std::size_t>(g.n[0]) * (c1 + static_cast<std::size_t>(g.n[1]) * c2);
+  for (int c0 = xbegin; c0 < g.n[0]; c0 += xstride) {
+    const int x0 = job.cover.start[0][c0], x1 = job.cover.start[0][c0 + 1];
+    T ar = T(0), ai = T(0);
+    if (yz_fast && x1 - x0 <= 2) {
+      int2 ex[2];
+      ex[0] = job.cover.entry[0][x0];
+      ex[1] = x1 - x0 == 2 ? job.cover.entry[0][x0 + 1] : make_int2(0, 0);
+      uint32_t s0[2 * NY * NZ], s1[2 * NY * NZ];
+#pragma unroll
+      for (int k = 0; k < 2 * NY * NZ; ++k) {
+        const int jx = k & 1, kr = k >> 1;
+        const bool valid = vrow[kr] && x0 + jx < x1;
+        const int bin = brow[kr] + ex[jx].x;
+        s0[k] = valid ? job.sub_start[bin] : 0u;
+        s1[k] = valid ? job.sub_start[bin + 1] : 0u;
+      }
+      // first subproblem of every combination: independent loads
+      Cplx<T> v[2 * NY * NZ];
+#pragma unroll
+      for (int k = 0; k < 2 * NY * NZ; ++k) {
+        v[k] = Cplx<T>{T(0), T(0)};
+        if (s1[k] > s0[k]) {
+          v[k] = job.tiles[lrow[k >> 1] + ex[k & 1].y + static_cast<std::size_t>(s0[k]) * tc];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 2 * NY * NZ; ++k) {
+        ar += v[k].x;
+        ai += v[k].y;
+      }
+      // bins split into several subproblems
+#pragma unroll
+      for (int k = 0; k < 2 * NY * NZ; ++k) {
+        const Cplx<T>* t = job.tiles + lrow[k >> 1] + ex[k & 1].y;
+        for (uint32_t s = s0[k] + 1; s < s1[k]; ++s) {
+          const Cplx<T> w = t[static_cast<std::size_t>(s) * tc];
+          ar += w.x;
+          ai += w.y;
+        }
+      }
+    } else {
+      reduce_cell_loops<T, DIM>(job, x0, x1, y0, y1, z0, z1, ar, ai);
+    }
+    if (accumulate) {
+      const Cplx<T> v = grow[c0];
+      ar += v.x;
+      ai += v.y;
+    }
+    grow[c0] = Cplx<T>{ar, ai};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bin-owned tile reduction for regular geometries (per dimension: bin >= ns,
+// at least two bins, last bin full or >= ns wide). Then a cell of bin b is
+// covered only by the tiles of b and of its two neighbours: by b at position
+// i + ns/2 (i = offset inside the bin), by b - 1 when i < ns - ns/2 (at
+// i + w(b - 1) + ns/2) and by b + 1 when i >= w(b) - ns/2 (at
+// i - w(b) + ns/2), w = bin width, neighbours periodic. One workgroup takes
+// one bin: the subproblem ranges of the 3^DIM neighbour bins go to LDS once,
+// and every tile load afterwards is independent of the others.
+// ---------------------------------------------------------------------------
+constexpr int kBinReduceThreads = 256;
+
+template <int DIM>
+constexpr int kNeigh = DIM == 1 ? 3 : (DIM == 2 ? 9 : 27);
+
+template <typename T, int DIM>
+__device__ inline void reduce_bin(const ReduceJob<T, DIM>& job, int ns, uint32_t bin,
+                                  uint32_t (*rng)[2], bool accumulate) {
+  const BinGeom& g = job.g;
+  const int h = ns / 2;
+  int b[3] = {0, 0, 0}, w[3] = {1, 1, 1}, wprev[3] = {1, 1, 1};
+  uint32_t rem = bin;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    b[d] = static_cast<int>(rem % static_cast<uint32_t>(g.nbins[d]));
+    rem /= static_cast<uint32_t>(g.nbins[d]);
+    w[d] = min(g.bin[d], g.n[d] - b[d] * g.bin[d]);
+    const int bp = b[d] == 0 ? g.nbins[d] - 1 : b[d] - 1;
+    wprev[d] = min(g.bin[d], g.n[d] - bp * g.bin[d]);
+  }
+  // subproblem ranges of the neighbour bins (offset -1, 0, +1 per dimension)
+  if (threadIdx.x < kNeigh<DIM>) {
+    int k = threadIdx.x, nb = 0, mult = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int off = k % 3 - 1;
+      k /= 3;
+      int bd = b[d] + off;
+      bd = bd < 0 ? bd + g.nbins[d] : (bd >= g.nbins[d] ? bd - g.nbins[d] : bd);
+      nb += bd * mult;
+      mult *= g.nbins[d];
+    }
+    rng[threadIdx.x][0] = job.sub_start[nb];
+    rng[threadIdx.x][1] = job.sub_start[nb + 1];
+  }
+  __syncthreads();
+  const std::size_t tc = tile_cells(g);
+  const int cells = w[0] * w[1] * w[2];
+  for (int i = threadIdx.x; i < cells; i += kBinReduceThreads) {
+    int li[3] = {i % w[0], DIM >= 2 ? (i / w[0]) % w[1] : 0, DIM >= 3 ? i / (w[0] * w[1]) : 0};
+    // per dimension: tile position for offsets -1, 0, +1 (or -1 if not covered)
+    int pos[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d < DIM) {
+        pos[d][0] = li[d] < ns - h ? li[d] + wprev[d] + h : -1;
+        pos[d][1] = li[d] + h;
+        pos[d][2] = li[d] >= w[d] - h ? li[d] - w[d] + h : -1;
+      } else {
+        pos[d][0] = -1;
+        pos[d][1] = 0;
+        pos[d][2] = -1;
+      }
+    }
+    // first subproblem of every covering neighbour: one z layer of (up to
+    // nine) independent loads at a time, then bins split into several
+    // subproblems
+    constexpr int kLayer = DIM == 1 ? 3 : 9;
+    T ar = T(0), ai = T(0);
+#pragma unroll
+    for (int kz = 0; kz < kNeigh<DIM> / kLayer; ++kz) {
+      Cplx<T> v[kLayer];
+#pragma unroll
+      for (int j = 0; j < kLayer; ++j) {
+        const int k = kz * kLayer + j;
+        const int px = pos[0][k % 3], py = pos[1][DIM >= 2 ? (k / 3) % 3 : 1];
+        const int pz = pos[2][DIM >= 3 ? k / 9 : 1];
+        v[j] = Cplx<T>{T(0), T(0)};
+        if (px >= 0 && py >= 0 && pz >= 0 && rng[k][1] > rng[k][0]) {
+          const int tp = px + g.tile[0] * (py + g.tile[1] * pz);
+          v[j] = job.tiles[static_cast<std::size_t>(tp) + static_cast<std::size_t>(rng[k][0]) * tc];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kLayer; ++j) {
+        ar += v[j].x;
+        ai += v[j].y;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kNeigh<DIM>; ++k) {
+      const int px = pos[0][k % 3], py = pos[1][DIM >= 2 ? (k / 3) % 3 : 1];
+      const int pz = pos[2][DIM >= 3 ? k / 9 : 1];
+      if (px >= 0 && py >= 0 && pz >= 0) {
+        const int tp = px + g.tile[0] * (py + g.tile[1] * pz);
+        for (uint32_t s = rng[k][0] + 1; s < rng[k][1]; ++s) {
+          const Cplx<T> w = job.tiles[static_cast<std::size_t>(tp) + static_cast<std::size_t>(s) * tc];
+          ar += w.x;
+          ai += w.y;
+        }
+      }
+    }
+    const std::size_t c =
+        static_cast<std::size_t>(b[0] * g.bin[0] + li[0]) +
+        static_cast<std::size_t>(g.n[0]) *
+            (static_cast<std::size_t>(b[1] * g.bin[1] + li[1]) +
+             static_cast<std::size_t>(g.n[1]) * static_cast<std::size_t>(b[2] * g.bin[2] + li[2]));
+    if (accumulate) {
+      const Cplx<T> v = job.grid[c];
+      ar += v.x;
+      ai += v.y;
+    }
+    job.grid[c] = Cplx<T>{ar, ai};
+  }
+  __syncthreads();  // rng is reused by the next bin
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kBinReduceThreads)
+    bin_reduce_kernel(ReduceJob<T, DIM> job, int ns, int accumulate) {
+  __shared__ uint32_t rng[kNeigh<DIM>][2];
+  const uint32_t nbins = static_cast<uint32_t>(job.g.nbins[0]) * job.g.nbins[1] * job.g.nbins[2];
+  // neighbouring bins on one XCD so their shared tile lines meet in one L2
+  for (uint32_t bin = xcd_remap(blockIdx.x, gridDim.x); bin < nbins; bin += gridDim.x) {
+    reduce_bin<T, DIM>(job, ns, bin, rng, accumulate != 0);
+  }
+}
+
+// blockIdx.y selects the job.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kBinReduceThreads)
+    bin_reduce_batch_kernel(const ReduceJob<T, DIM>* __restrict__ jobs, int ns, int accumulate) {
+  __shared__ uint32_t rng[kNeigh<DIM>][2];
+  const ReduceJob<T, DIM> job = jobs[blockIdx.y];
+  const uint32_t nbins = static_cast<uint32_t>(job.g.nbins[0]) * job.g.nbins[1] * job.g.nbins[2];
+  for (uint32_t bin = blockIdx.x; bin < nbins; bin += gridDim.x) {
+    reduce_bin<T, DIM>(job, ns, bin, rng, accumulate != 0);
+  }
+}
+
+// blockIdx.y strides over the rows (y, z) of the grid, blockIdx.x over x.
+template <typename T, int DIM>
+__device__ inline void reduce_grid(const ReduceJob<T, DIM>& job, bool accumulate) {
+  const int rows = job.g.n[1] * job.g.n[2];
+  const int xbegin = threadIdx.x + blockIdx.x * kReduceThreads;
+  const int xstride = gridDim.x * kReduceThreads;
+  for (int row = blockIdx.y; row < rows; row += gridDim.y) {
+    const int c1 = DIM >= 2 ? row % job.g.n[1] : 0;
+    const int c2 = DIM >= 3 ? row / job.g.n[1] : 0;
+    reduce_row<T, DIM>(job, c1, c2, xbegin, xstride, accumulate);
+  }
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kReduceThreads)
+    tile_reduce_kernel(ReduceJob<T, DIM> job, int accumulate) {
+  reduce_grid<T, DIM>(job, accumulate != 0);
+}
+
+// blockIdx.z selects the job.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kReduceThreads)
+    tile_reduce_batch_kernel(const ReduceJob<T, DIM>* __restrict__ jobs, int accumulate) {
+  const ReduceJob<T, DIM> job = jobs[blockIdx.z];
+  reduce_grid<T, DIM>(job, accumulate != 0);
+}
+
+// Launch shape of a reduction over grids with x extent n0 and `rows` rows:
+// enough x blocks to cover a row (at most 8 unless there are few rows, as in
+// 1D), rows spread over y.
+inline dim3 reduce_shape(int n0, int rows, unsigned zcount, unsigned max_y) {
+  unsigned gx = static_cast<unsigned>((n0 + kReduceThreads - 1) / kReduceThreads);
+  const unsigned gx_cap = rows < 64 ? 4096u : 8u;
+  if (gx > gx_cap) gx = gx_cap;
+  unsigned gy = static_cast<unsigned>(rows);
+  if (gy > max_y) gy = max_y;
+  if (gy < 1u) gy = 1u;
+  return dim3(gx, gy, zcount);
+}
+
 }  // namespace
 
 template <typename T>
@@ -473,10 +784,10 @@ std::size_t tile_bytes(const BinGeom& g) {
 
 template <typename T, int DIM>
 void launch_spread(int ns, const PointRec<T, DIM>* pts, const uint32_t* perm, const Subprob* subs,
-                   std::size_t nsub, const Cplx<T>* in, const Cplx<T>* pre_sorted, Cplx<T>* grid,
+                   std::size_t nsub, const Cplx<T>* in, const Cplx<T>* pre_sorted, Cplx<T>* tiles,
                    const BinGeom& g, const EsParams& es, hipStream_t stream) {
   if (nsub == 0) return;
-  const SpreadJob<T, DIM> job{pts, perm, in, pre_sorted, grid, g};
+  const SpreadJob<T, DIM> job{pts, perm, in, pre_sorted, tiles, 0u, 0u, g};
   NNF_NS_CASES(spread_ns, job, subs, nsub, es, stream)
 }
 
@@ -505,6 +816,44 @@ void launch_interp_batch(int ns, const InterpJob<T, DIM>* jobs, const Subprob* s
   NNF_NS_CASES(interp_batch_ns, jobs, subs, nsub, lds_bytes, es, stream)
 }
 
+template <typename T, int DIM>
+void launch_tile_reduce(const ReduceJob<T, DIM>& job, int ns, bool accumulate,
+                        hipStream_t stream) {
+  if (bins_regular(DIM, job.g, ns)) {
+    const std::size_t nbins =
+        static_cast<std::size_t>(job.g.nbins[0]) * job.g.nbins[1] * job.g.nbins[2];
+    const unsigned blocks = static_cast<unsigned>(std::min<std::size_t>(nbins, 65535u * 16u));
+    hipLaunchKernelGGL((bin_reduce_kernel<T, DIM>), dim3(blocks), dim3(kBinReduceThreads), 0,
+                       stream, job, ns, accumulate ? 1 : 0);
+    check_launch("bin_reduce_kernel");
+    return;
+  }
+  const dim3 shape = reduce_shape(job.g.n[0], job.g.n[1] * job.g.n[2], 1u, 65535u);
+  hipLaunchKernelGGL((tile_reduce_kernel<T, DIM>), shape, dim3(kReduceThreads), 0, stream, job,
+                     accumulate ? 1 : 0);
+  check_launch("tile_reduce_kernel");
+}
+
+template <typename T, int DIM>
+void launch_tile_reduce_batch(const ReduceJob<T, DIM>* jobs, int count, int ns, bool regular,
+                              std::size_t max_bins, int max_n0, int max_rows, bool accumulate,
+                              hipStream_t stream) {
+  if (count <= 0) return;
+  if (count > 65535) throw neonufft::InputError("Neonufft:: too many transforms in one batch");
+  if (regular) {
+    const unsigned blocks = static_cast<unsigned>(std::min<std::size_t>(max_bins, 4096u));
+    hipLaunchKernelGGL((bin_reduce_batch_kernel<T, DIM>),
+                       dim3(blocks, static_cast<unsigned>(count)), dim3(kBinReduceThreads), 0,
+                       stream, jobs, ns, accumulate ? 1 : 0);
+    check_launch("bin_reduce_batch_kernel");
+    return;
+  }
+  const dim3 shape = reduce_shape(max_n0, max_rows, static_cast<unsigned>(count), 1024u);
+  hipLaunchKernelGGL((tile_reduce_batch_kernel<T, DIM>), shape, dim3(kReduceThreads), 0, stream,
+                     jobs, accumulate ? 1 : 0);
+  check_launch("tile_reduce_batch_kernel");
+}
+
 #define NNF_INST_SI(T, DIM)                                                                      \
   template void launch_spread<T, DIM>(int, const PointRec<T, DIM>*, const uint32_t*,             \
                                       const Subprob*, std::size_t, const Cplx<T>*,               \
@@ -519,7 +868,10 @@ void launch_interp_batch(int ns, const InterpJob<T, DIM>* jobs, const Subprob* s
                                             hipStream_t);                                        \
   template void launch_interp_batch<T, DIM>(int, const InterpJob<T, DIM>*, const Subprob*,       \
                                             std::size_t, std::size_t, const EsParams&,           \
-                                            hipStream_t);
+                                            hipStream_t);                                        \
+  template void launch_tile_reduce<T, DIM>(const ReduceJob<T, DIM>&, int, bool, hipStream_t);   \
+  template void launch_tile_reduce_batch<T, DIM>(const ReduceJob<T, DIM>*, int, int, bool,       \
+                                                 std::size_t, int, int, bool, hipStream_t);
 
 NNF_INST_SI(float, 1)
 NNF_INST_SI(float, 2)
