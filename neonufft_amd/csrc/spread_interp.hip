@@ -623,71 +623,97 @@ __device__ inline void reduce_bin(const ReduceJob<T, DIM>& job, int ns, uint32_t
   __syncthreads();
   const std::size_t tc = tile_cells(g);
   const int cells = w[0] * w[1] * w[2];
-  for (int i = threadIdx.x; i < cells; i += kBinReduceThreads) {
-    int li[3] = {i % w[0], DIM >= 2 ? (i / w[0]) % w[1] : 0, DIM >= 3 ? i / (w[0] * w[1]) : 0};
-    // per dimension: tile position for offsets -1, 0, +1 (or -1 if not covered)
-    int pos[3][3];
+  constexpr int kCombos = 1 << DIM;  // own tile or the one covering neighbour, per dimension
+  constexpr int kCells = 2;          // cells per thread per pass (loads in flight together)
+  for (int ibase = threadIdx.x; ibase < cells; ibase += kCells * kBinReduceThreads) {
+    Cplx<T> v[kCells][kCombos];
+    int tpos[kCells][kCombos], nidx[kCells][kCombos];
+    bool ok[kCells][kCombos];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (d < DIM) {
-        pos[d][0] = li[d] < ns - h ? li[d] + wprev[d] + h : -1;
-        pos[d][1] = li[d] + h;
-        pos[d][2] = li[d] >= w[d] - h ? li[d] - w[d] + h : -1;
-      } else {
-        pos[d][0] = -1;
-        pos[d][1] = 0;
-        pos[d][2] = -1;
-      }
-    }
-    // first subproblem of every covering neighbour: one z layer of (up to
-    // nine) independent loads at a time, then bins split into several
-    // subproblems
-    constexpr int kLayer = DIM == 1 ? 3 : 9;
-    T ar = T(0), ai = T(0);
+    for (int u = 0; u < kCells; ++u) {
+      const int i = ibase + u * kBinReduceThreads;
+      const bool live = i < cells;
+      const int li[3] = {i % w[0], DIM >= 2 ? (i / w[0]) % w[1] : 0,
+                         DIM >= 3 ? i / (w[0] * w[1]) : 0};
+      // per dimension: own position, and the covering neighbour (-1, +1 or
+      // none = 0) with its position; regular bins never have both
+      int pown[3] = {0, 0, 0}, poth[3] = {0, 0, 0}, ooth[3] = {0, 0, 0};
 #pragma unroll
-    for (int kz = 0; kz < kNeigh<DIM> / kLayer; ++kz) {
-      Cplx<T> v[kLayer];
-#pragma unroll
-      for (int j = 0; j < kLayer; ++j) {
-        const int k = kz * kLayer + j;
-        const int px = pos[0][k % 3], py = pos[1][DIM >= 2 ? (k / 3) % 3 : 1];
-        const int pz = pos[2][DIM >= 3 ? k / 9 : 1];
-        v[j] = Cplx<T>{T(0), T(0)};
-        if (px >= 0 && py >= 0 && pz >= 0 && rng[k][1] > rng[k][0]) {
-          const int tp = px + g.tile[0] * (py + g.tile[1] * pz);
-          v[j] = job.tiles[static_cast<std::size_t>(tp) + static_cast<std::size_t>(rng[k][0]) * tc];
+      for (int d = 0; d < DIM; ++d) {
+        pown[d] = li[d] + h;
+        if (li[d] < ns - h) {
+          ooth[d] = -1;
+          poth[d] = li[d] + wprev[d] + h;
+        } else if (li[d] >= w[d] - h) {
+          ooth[d] = 1;
+          poth[d] = li[d] - w[d] + h;
         }
       }
 #pragma unroll
-      for (int j = 0; j < kLayer; ++j) {
-        ar += v[j].x;
-        ai += v[j].y;
+      for (int k = 0; k < kCombos; ++k) {
+        bool good = live;
+        int p[3] = {0, 0, 0}, nb = 0, mult = 1;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const bool other = (k >> d) & 1;
+          good = good && (!other || ooth[d] != 0);
+          p[d] = other ? poth[d] : pown[d];
+          nb += ((other ? ooth[d] : 0) + 1) * mult;
+          mult *= 3;
+        }
+        ok[u][k] = good;
+        nidx[u][k] = nb;
+        tpos[u][k] = p[0] + g.tile[0] * (p[1] + g.tile[1] * p[2]);
       }
     }
 #pragma unroll
-    for (int k = 0; k < kNeigh<DIM>; ++k) {
-      const int px = pos[0][k % 3], py = pos[1][DIM >= 2 ? (k / 3) % 3 : 1];
-      const int pz = pos[2][DIM >= 3 ? k / 9 : 1];
-      if (px >= 0 && py >= 0 && pz >= 0) {
-        const int tp = px + g.tile[0] * (py + g.tile[1] * pz);
-        for (uint32_t s = rng[k][0] + 1; s < rng[k][1]; ++s) {
-          const Cplx<T> w = job.tiles[static_cast<std::size_t>(tp) + static_cast<std::size_t>(s) * tc];
-          ar += w.x;
-          ai += w.y;
+    for (int u = 0; u < kCells; ++u) {
+#pragma unroll
+      for (int k = 0; k < kCombos; ++k) {
+        v[u][k] = Cplx<T>{T(0), T(0)};
+        if (ok[u][k]) {
+          const uint32_t s0 = rng[nidx[u][k]][0];
+          if (rng[nidx[u][k]][1] > s0) {
+            v[u][k] = job.tiles[static_cast<std::size_t>(tpos[u][k]) +
+                                static_cast<std::size_t>(s0) * tc];
+          }
         }
       }
     }
-    const std::size_t c =
-        static_cast<std::size_t>(b[0] * g.bin[0] + li[0]) +
-        static_cast<std::size_t>(g.n[0]) *
-            (static_cast<std::size_t>(b[1] * g.bin[1] + li[1]) +
-             static_cast<std::size_t>(g.n[1]) * static_cast<std::size_t>(b[2] * g.bin[2] + li[2]));
-    if (accumulate) {
-      const Cplx<T> v = job.grid[c];
-      ar += v.x;
-      ai += v.y;
+#pragma unroll
+    for (int u = 0; u < kCells; ++u) {
+      const int i = ibase + u * kBinReduceThreads;
+      if (i >= cells) continue;
+      T ar = T(0), ai = T(0);
+#pragma unroll
+      for (int k = 0; k < kCombos; ++k) {
+        ar += v[u][k].x;
+        ai += v[u][k].y;
+        if (ok[u][k]) {  // bins split into several subproblems
+          const uint32_t s1 = rng[nidx[u][k]][1];
+          for (uint32_t s = rng[nidx[u][k]][0] + 1; s < s1; ++s) {
+            const Cplx<T> x = job.tiles[static_cast<std::size_t>(tpos[u][k]) +
+                                        static_cast<std::size_t>(s) * tc];
+            ar += x.x;
+            ai += x.y;
+          }
+        }
+      }
+      const int li[3] = {i % w[0], DIM >= 2 ? (i / w[0]) % w[1] : 0,
+                         DIM >= 3 ? i / (w[0] * w[1]) : 0};
+      const std::size_t c =
+          static_cast<std::size_t>(b[0] * g.bin[0] + li[0]) +
+          static_cast<std::size_t>(g.n[0]) *
+              (static_cast<std::size_t>(b[1] * g.bin[1] + li[1]) +
+               static_cast<std::size_t>(g.n[1]) *
+                   static_cast<std::size_t>(b[2] * g.bin[2] + li[2]));
+      if (accumulate) {
+        const Cplx<T> a = job.grid[c];
+        ar += a.x;
+        ai += a.y;
+      }
+      job.grid[c] = Cplx<T>{ar, ai};
     }
-    job.grid[c] = Cplx<T>{ar, ai};
   }
   __syncthreads();  // rng is reused by the next bin
 }
