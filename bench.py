@@ -542,8 +542,10 @@ def run_uniform(args):
     reps = args.kernel_reps
     t1_ms = timed(lambda: plan.transform_type_1(dc, f), reps)
     t2_ms = timed(lambda: plan.transform_type_2(f, None, out2), reps)
-    # dominant kernels alone: spread (accumulating, no zero fill) and interp
-    spread_ms = timed(lambda: plan.spread(dc, accumulate=True), reps)
+    # the kernels alone: spreading into subgrid tiles, the tile reduction that
+    # forms the fine grid, and interpolation
+    spread_ms = timed(lambda: plan.spread_tiles(dc), reps)
+    reduce_ms = timed(lambda: plan.reduce_tiles(accumulate=True), reps)
     interp_ms = timed(lambda: plan.interp(out2), reps)
 
     # restore valid outputs for the accuracy check
@@ -565,6 +567,11 @@ def run_uniform(args):
     alg_bytes = m * (3 * 4 + 8) + G * 8  # coords + strength, fine grid once
     spread_gbs = alg_bytes / (spread_ms * 1e-3)
     interp_gbs = alg_bytes / (interp_ms * 1e-3)
+    # bytes the reduction moves: every stored tile cell read once, grid written
+    tile_cells = 1
+    for d, nb in enumerate(info["bin_size"]):
+        tile_cells *= nb + info["n_spread"]
+    reduce_bytes = info["num_subproblems"] * tile_cells * 8 + G * 8
     dom = ("spread_kernel", spread_ms, spread_gbs) if spread_ms >= interp_ms else \
         ("interp_kernel", interp_ms, interp_gbs)
 
@@ -625,6 +632,9 @@ def run_uniform(args):
         },
         "kernels": {
             "spread_ms": spread_ms, "spread_GBps": spread_gbs / 1e9,
+            "reduce_ms": reduce_ms, "reduce_GBps": reduce_bytes / (reduce_ms * 1e-3) / 1e9,
+            "reduce_bytes": reduce_bytes,
+            "spread_stage_ms": spread_ms + reduce_ms,
             "interp_ms": interp_ms, "interp_GBps": interp_gbs / 1e9,
         },
         "stages": {

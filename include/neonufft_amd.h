@@ -183,9 +183,15 @@ typedef struct nnf_plan_info {
 int nnf_plan_get_info(nnf_plan plan, nnf_plan_info* info);
 int nnf_plan_t3_get_info(nnf_plan_t3 plan, nnf_plan_info* info);
 
-/* Spread `in` onto the fine grid (type-1 stage 1). The grid is zeroed first
- * unless `accumulate` is non-zero. */
+/* Spread `in` onto the fine grid (type-1 stage 1): the grid is overwritten,
+ * or added to when `accumulate` is non-zero. Equals nnf_plan_spread_tiles
+ * followed by nnf_plan_reduce_tiles. */
 int nnf_plan_spread(nnf_plan plan, const void* in, int accumulate);
+/* The spreading kernel alone: every subproblem's subgrid tile into the plan's
+ * tile buffer. */
+int nnf_plan_spread_tiles(nnf_plan plan, const void* in);
+/* The tile reduction alone: fine grid (+)= sum of the covering tiles. */
+int nnf_plan_reduce_tiles(nnf_plan plan, int accumulate);
 /* Interpolate the current fine grid at the plan's points into `out`
  * (type-2 stage 3). */
 int nnf_plan_interp(nnf_plan plan, void* out);

@@ -235,6 +235,14 @@ class Plan:
     def spread(self, in_: Pointer, accumulate: bool = False) -> None:
         _check(self._L.nnf_plan_spread(self._h, ctypes.c_void_p(_ptr(in_)), int(accumulate)))
 
+    def spread_tiles(self, in_: Pointer) -> None:
+        """The spreading kernel alone (subgrid tiles; see spread())."""
+        _check(self._L.nnf_plan_spread_tiles(self._h, ctypes.c_void_p(_ptr(in_))))
+
+    def reduce_tiles(self, accumulate: bool = False) -> None:
+        """The tile reduction alone: fine grid (+)= sum of covering tiles."""
+        _check(self._L.nnf_plan_reduce_tiles(self._h, int(accumulate)))
+
     def interp(self, out: Pointer) -> None:
         _check(self._L.nnf_plan_interp(self._h, ctypes.c_void_p(_ptr(out))))
 

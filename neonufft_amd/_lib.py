@@ -19,7 +19,7 @@ EXPORTED = (
     "nnf_plan_fine_grid", "nnf_plan_fseries", "nnf_plan_create_tuned",
     "nnf_batch_create", "nnf_batch_set_points", "nnf_batch_transform_type_1",
     "nnf_batch_transform_type_2", "nnf_batch_get_info", "nnf_batch_destroy",
-    "nnf_autotune", "nnf_autotune_clear_cache",
+    "nnf_autotune", "nnf_autotune_clear_cache", "nnf_plan_spread_tiles", "nnf_plan_reduce_tiles",
 )
 
 
@@ -124,6 +124,8 @@ def lib():
     L.nnf_plan_get_info.argtypes = [vp, ctypes.POINTER(NnfPlanInfo)]
     L.nnf_plan_t3_get_info.argtypes = [vp, ctypes.POINTER(NnfPlanInfo)]
     L.nnf_plan_spread.argtypes = [vp, vp, c_int]
+    L.nnf_plan_spread_tiles.argtypes = [vp, vp]
+    L.nnf_plan_reduce_tiles.argtypes = [vp, c_int]
     L.nnf_plan_interp.argtypes = [vp, vp]
     L.nnf_plan_fine_grid.argtypes = [vp, vpp]
     L.nnf_plan_fseries.argtypes = [vp, c_int, vpp]

@@ -39,6 +39,7 @@ SOURCES = [
     ("points.hip", True),
     ("spread_interp.hip", True),
     ("grid.hip", True),
+    ("pruned.hip", True),
     ("params.cpp", False),
     ("fft.cpp", False),
     ("plan.cpp", False),

@@ -137,8 +137,12 @@ public:
     }
     grids_.reset(off, alloc_);
     for (Group& g : groups_) {
-      g.fft = FftPlan(std::is_same_v<T, double>, DIM, g.n, sign_,
-                      static_cast<IntType>(g.members.size()), stream_, alloc_);
+      if constexpr (DIM == 1) {
+        g.fft = FftPlan(std::is_same_v<T, double>, DIM, g.n, sign_,
+                        static_cast<IntType>(g.members.size()), stream_, alloc_);
+      } else {
+        g.pfft.init(g.n, g.modes, sign_, static_cast<int>(g.members.size()), stream_, alloc_);
+      }
       for (int d = 0; d < DIM; ++d) {
         g.fs[d].reset(static_cast<std::size_t>(g.n[d] / 2 + 1), alloc_);
         launch_fseries<T>(g.fs[d].data(), g.n[d], ks_.quad, stream_);
@@ -271,7 +275,8 @@ private:
     IntType n[3];
     std::size_t cells = 0, offset = 0;
     std::vector<int> members;
-    FftPlan fft;
+    FftPlan fft;                            // 1D groups
+    PrunedFft<T, DIM == 1 ? 2 : DIM> pfft;  // 2D / 3D groups
     DeviceBuffer<T> fs[3];
     BinGeom geom{};
     CoverBuffers cover;
@@ -296,24 +301,36 @@ private:
       launch_tile_reduce_batch<T, DIM>(rjobs_.data(), count_, ks_.ns, regular_, max_bins_,
                                        max_n0_, max_rows_, false, s);
       for (Group& g : groups_) {
-        g.fft.set_stream(s);
-        g.fft.execute(grids_.data() + g.offset);
         T* fs[3] = {g.fs[0].data(), g.fs[1].data(), g.fs[2].data()};
         IntType st[3] = {1, g.modes[0], g.modes[0] * g.modes[1]};
-        launch_downsample_batch<T, DIM>(order_fft(), grids_.data() + g.offset, g.cells, g.n, fs,
-                                        reinterpret_cast<Cplx<T>* const*>(io_.data() + g.io_off), st,
-                                        g.modes, static_cast<int>(g.members.size()), s);
+        Cplx<T>* const* outs = reinterpret_cast<Cplx<T>* const*>(io_.data() + g.io_off);
+        if constexpr (DIM == 1) {
+          g.fft.set_stream(s);
+          g.fft.execute(grids_.data() + g.offset);
+          launch_downsample_batch<T, DIM>(order_fft(), grids_.data() + g.offset, g.cells, g.n,
+                                          fs, outs, st, g.modes,
+                                          static_cast<int>(g.members.size()), s);
+        } else {
+          g.pfft.set_stream(s);
+          g.pfft.forward(grids_.data() + g.offset, fs, order_fft(), nullptr, outs, st, s);
+        }
       }
     } else {
       for (Group& g : groups_) {
         T* fs[3] = {g.fs[0].data(), g.fs[1].data(), g.fs[2].data()};
         IntType st[3] = {1, g.modes[0], g.modes[0] * g.modes[1]};
-        launch_upsample_batch<T, DIM>(order_fft(),
-                                      reinterpret_cast<const Cplx<T>* const*>(io_.data() + g.io_off), st,
-                                      g.modes, fs, grids_.data() + g.offset, g.cells, g.n,
-                                      static_cast<int>(g.members.size()), s);
-        g.fft.set_stream(s);
-        g.fft.execute(grids_.data() + g.offset);
+        const Cplx<T>* const* ins =
+            reinterpret_cast<const Cplx<T>* const*>(io_.data() + g.io_off);
+        if constexpr (DIM == 1) {
+          launch_upsample_batch<T, DIM>(order_fft(), ins, st, g.modes, fs,
+                                        grids_.data() + g.offset, g.cells, g.n,
+                                        static_cast<int>(g.members.size()), s);
+          g.fft.set_stream(s);
+          g.fft.execute(grids_.data() + g.offset);
+        } else {
+          g.pfft.set_stream(s);
+          g.pfft.backward(nullptr, ins, st, fs, order_fft(), grids_.data() + g.offset, s);
+        }
       }
       launch_interp_batch<T, DIM>(ks_.ns, ijobs_.data(), subs_.data(), nsub_, max_lds_, ks_.es, s);
     }
@@ -343,7 +360,13 @@ private:
     }
     const hipError_t end = hipStreamEndCapture(capture_stream_, &graph);
     (void)hipGetLastError();
-    for (Group& g : groups_) g.fft.set_stream(stream_);
+    for (Group& g : groups_) {
+      if constexpr (DIM == 1) {
+        g.fft.set_stream(stream_);
+      } else {
+        g.pfft.set_stream(stream_);
+      }
+    }
     if (ok && end == hipSuccess && graph) {
       if (hipGraphInstantiate(&exec_[k], graph, nullptr, nullptr, 0) == hipSuccess) {
         graph_ok_[k] = true;

@@ -31,6 +31,8 @@ struct PlanIface {
   virtual void t1(const void* in, void* out, const int64_t* st) = 0;
   virtual void t2(const void* in, const int64_t* st, void* out) = 0;
   virtual void spread(const void* in, bool accumulate) = 0;
+  virtual void spread_tiles(const void* in) = 0;
+  virtual void reduce_tiles(bool accumulate) = 0;
   virtual void interp(void* out) = 0;
   virtual void* grid() = 0;
   virtual const void* fseries(int d) = 0;
@@ -56,6 +58,10 @@ struct PlanHolder final : PlanIface {
   void spread(const void* in, bool accumulate) override {
     impl.spread(static_cast<const Cplx<T>*>(in), accumulate);
   }
+  void spread_tiles(const void* in) override {
+    impl.spread_tiles(static_cast<const Cplx<T>*>(in));
+  }
+  void reduce_tiles(bool accumulate) override { impl.reduce_tiles(accumulate); }
   void interp(void* out) override { impl.interp(static_cast<Cplx<T>*>(out)); }
   void* grid() override { return impl.fine_grid(); }
   const void* fseries(int d) override { return impl.fseries(d); }
@@ -430,6 +436,20 @@ NEONUFFT_EXPORT int nnf_plan_spread(nnf_plan plan, const void* in, int accumulat
   return guarded([&] {
     NNF_REQUIRE_PLAN(plan);
     plan->p->spread(in, accumulate != 0);
+  });
+}
+
+NEONUFFT_EXPORT int nnf_plan_spread_tiles(nnf_plan plan, const void* in) {
+  return guarded([&] {
+    NNF_REQUIRE_PLAN(plan);
+    plan->p->spread_tiles(in);
+  });
+}
+
+NEONUFFT_EXPORT int nnf_plan_reduce_tiles(nnf_plan plan, int accumulate) {
+  return guarded([&] {
+    NNF_REQUIRE_PLAN(plan);
+    plan->p->reduce_tiles(accumulate != 0);
   });
 }
 

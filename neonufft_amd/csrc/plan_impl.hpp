@@ -20,6 +20,7 @@
 #include "fft.hpp"
 #include "params.hpp"
 #include "point_set.hpp"
+#include "pruned_fft.hpp"
 
 namespace nnf {
 
@@ -51,7 +52,11 @@ public:
       cells *= static_cast<std::size_t>(n_[d]);
     }
     grid_.reset(cells, alloc_);
-    fft_ = FftPlan(std::is_same_v<T, double>, DIM, n_, sign_, 1, stream_, alloc_);
+    if constexpr (DIM == 1) {
+      fft_ = FftPlan(std::is_same_v<T, double>, DIM, n_, sign_, 1, stream_, alloc_);
+    } else {
+      pfft_.init(n_, modes_, sign_, 1, stream_, alloc_);
+    }
     for (int d = 0; d < DIM; ++d) {
       fs_[d].reset(static_cast<std::size_t>(n_[d] / 2 + 1), alloc_);
       launch_fseries<T>(fs_[d].data(), n_[d], ks_.quad, stream_);
@@ -75,9 +80,17 @@ public:
   // Stage 1 of type 1: spread into per-subproblem tiles, then form the fine
   // grid from them (overwriting it, or adding to it when accumulating).
   void spread(const Cplx<T>* in, bool accumulate = false) {
+    spread_tiles(in);
+    reduce_tiles(accumulate);
+  }
+
+  // The two kernels of spread(), separately (for stage timing).
+  void spread_tiles(const Cplx<T>* in) {
     if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
     launch_spread<T, DIM>(ks_.ns, pts_.recs(), pts_.perm(), pts_.subs(), pts_.nsub(), in, nullptr,
                           tiles_.data(), geom_, ks_.es, stream_);
+  }
+  void reduce_tiles(bool accumulate) {
     const ReduceJob<T, DIM> job{tiles_.data(), pts_.sub_starts(), grid_.data(), geom_,
                                 cover_.table()};
     launch_tile_reduce<T, DIM>(job, ks_.ns, accumulate, stream_);
@@ -93,16 +106,25 @@ public:
   void transform_type_1(const Cplx<T>* in, Cplx<T>* out, const IntType* out_strides) {
     if (!out || !out_strides) throw neonufft::InputError("Neonufft:: null output");
     spread(in);
-    fft_.execute(grid_.data());
     T* fs[3] = {fs_[0].data(), fs_[1].data(), fs_[2].data()};
-    launch_downsample<T, DIM>(order_fft(), grid_.data(), n_, fs, out, out_strides, modes_, stream_);
+    if constexpr (DIM == 1) {
+      fft_.execute(grid_.data());
+      launch_downsample<T, DIM>(order_fft(), grid_.data(), n_, fs, out, out_strides, modes_,
+                                stream_);
+    } else {
+      pfft_.forward(grid_.data(), fs, order_fft(), out, nullptr, out_strides, stream_);
+    }
   }
 
   void transform_type_2(const Cplx<T>* in, const IntType* in_strides, Cplx<T>* out) {
     if (!in || !in_strides) throw neonufft::InputError("Neonufft:: null input");
     T* fs[3] = {fs_[0].data(), fs_[1].data(), fs_[2].data()};
-    launch_upsample<T, DIM>(order_fft(), in, in_strides, modes_, fs, grid_.data(), n_, stream_);
-    fft_.execute(grid_.data());
+    if constexpr (DIM == 1) {
+      launch_upsample<T, DIM>(order_fft(), in, in_strides, modes_, fs, grid_.data(), n_, stream_);
+      fft_.execute(grid_.data());
+    } else {
+      pfft_.backward(in, nullptr, in_strides, fs, order_fft(), grid_.data(), stream_);
+    }
     interp(out);
   }
 
@@ -143,7 +165,8 @@ private:
   DeviceBuffer<Cplx<T>> grid_;
   DeviceBuffer<Cplx<T>> tiles_;  // spreading tiles, tile_cells(geom_) per subproblem
   CoverBuffers cover_;
-  FftPlan fft_;
+  FftPlan fft_;                                // 1D plans
+  PrunedFft<T, DIM == 1 ? 2 : DIM> pfft_;      // 2D / 3D plans
   DeviceBuffer<T> fs_[3];
   PointSet<T, DIM> pts_;
 };

@@ -1,0 +1,142 @@
+// Pruned 2D / 3D FFT between a batch of fine grids and their kept modes
+// (type 1: FFT, truncate, deconvolve; type 2: deconvolve, zero-pad, FFT).
+//
+// Same results as the full in-place transform followed by downsampling (or
+// upsampling followed by it); the work is cut by transforming one axis at a
+// time with contiguous 1D rocFFT passes and carrying only the retained
+// entries of the axes already done:
+//   G[b][z][y][x] --x-FFT--> compact-transpose --> H[b][z][x'][y]
+//   --y-FFT--> compact-transpose --> K[b][y'][x'][z] --z-FFT--> modes
+// (2D stops after H). Type 2 runs the mirror image with zero-padding
+// transposes.
+#pragma once
+
+#include <memory>
+
+#include "fft.hpp"
+#include "kernels.hpp"
+
+namespace nnf {
+
+template <typename T>
+void launch_compact_transpose(const Cplx<T>* src, Cplx<T>* dst, int R, int nc, int Nc,
+                              long long src_rs, long long dst_rs, std::size_t slabs, int lo_count,
+                              long long src_hi, long long src_lo, long long dst_hi,
+                              long long dst_lo, hipStream_t stream);
+template <typename T>
+void launch_expand_transpose(const Cplx<T>* src, Cplx<T>* dst, int R, int nf, int Nf,
+                             long long src_rs, long long dst_rs, std::size_t slabs, int lo_count,
+                             long long src_hi, long long src_lo, long long dst_hi,
+                             long long dst_lo, hipStream_t stream);
+template <typename T, int DIM>
+void launch_modes_out(const Cplx<T>* src, long long member_stride, Cplx<T>* single,
+                      Cplx<T>* const* outs, int count, int N0, int Nm, int nl, int Nl,
+                      long long st0, long long stm, long long stl, const T* fs0, const T* fsm,
+                      const T* fsl, int order_fft, hipStream_t stream);
+template <typename T, int DIM>
+void launch_modes_in(const Cplx<T>* single, const Cplx<T>* const* ins, int count, Cplx<T>* dst,
+                     long long member_stride, int N0, int Nm, int nl, int Nl, long long st0,
+                     long long stm, long long stl, const T* fs0, const T* fsm, const T* fsl,
+                     int order_fft, hipStream_t stream);
+
+// DIM 2 or 3 (1D plans transform their single axis directly).
+template <typename T, int DIM>
+class PrunedFft {
+public:
+  PrunedFft() = default;
+  PrunedFft(const PrunedFft&) = delete;
+  PrunedFft& operator=(const PrunedFft&) = delete;
+  PrunedFft(PrunedFft&&) = default;
+  PrunedFft& operator=(PrunedFft&&) = default;
+
+  // n: fine grid, N: modes (fastest first); `count` grids, contiguous.
+  void init(const IntType* n, const IntType* N, int sign, int count, hipStream_t stream,
+            const std::shared_ptr<neonufft::Allocator>& alloc) {
+    for (int d = 0; d < 3; ++d) {
+      n_[d] = d < DIM ? static_cast<long long>(n[d]) : 1;
+      N_[d] = d < DIM ? static_cast<long long>(N[d]) : 1;
+    }
+    count_ = count;
+    const bool dbl = std::is_same_v<T, double>;
+    IntType len = n_[0];
+    px_ = FftPlan(dbl, 1, &len, sign, static_cast<IntType>(count * n_[1] * n_[2]), stream, alloc);
+    len = n_[1];
+    py_ = FftPlan(dbl, 1, &len, sign, static_cast<IntType>(count * n_[2] * N_[0]), stream, alloc);
+    h_.reset(static_cast<std::size_t>(count * n_[2] * N_[0] * n_[1]), alloc);
+    if (DIM == 3) {
+      len = n_[2];
+      pz_ = FftPlan(dbl, 1, &len, sign, static_cast<IntType>(count * N_[1] * N_[0]), stream, alloc);
+      k_.reset(static_cast<std::size_t>(count * N_[1] * N_[0] * n_[2]), alloc);
+    }
+  }
+
+  void set_stream(hipStream_t s) {
+    px_.set_stream(s);
+    py_.set_stream(s);
+    if (DIM == 3) pz_.set_stream(s);
+  }
+
+  // Type 1: grids (destroyed) -> deconvolved modes in the caller's order and
+  // strides; outputs `single` (count = 1) or the device table `outs`.
+  void forward(Cplx<T>* grids, T* const* fs, int order_fft, Cplx<T>* single,
+               Cplx<T>* const* outs, const IntType* st, hipStream_t s) {
+    const long long n0 = n_[0], n1 = n_[1], n2 = n_[2], N0 = N_[0], N1 = N_[1], N2 = N_[2];
+    const long long gstride = n0 * n1 * n2;
+    px_.execute(grids);
+    launch_compact_transpose<T>(grids, h_.data(), static_cast<int>(n1), static_cast<int>(n0),
+                                static_cast<int>(N0), n0, n1,
+                                static_cast<std::size_t>(count_ * n2), static_cast<int>(n2),
+                                gstride, n1 * n0, n2 * N0 * n1, N0 * n1, s);
+    py_.execute(h_.data());
+    if constexpr (DIM == 2) {
+      launch_modes_out<T, 2>(h_.data(), N0 * n1, single, outs, count_, static_cast<int>(N0), 1,
+                             static_cast<int>(n1), static_cast<int>(N1), st[0], 0, st[1], fs[0],
+                             fs[1], fs[1], order_fft, s);
+    } else {
+      launch_compact_transpose<T>(h_.data(), k_.data(), static_cast<int>(n2),
+                                  static_cast<int>(n1), static_cast<int>(N1), N0 * n1, N0 * n2,
+                                  static_cast<std::size_t>(count_ * N0), static_cast<int>(N0),
+                                  n2 * N0 * n1, n1, N1 * N0 * n2, n2, s);
+      pz_.execute(k_.data());
+      launch_modes_out<T, 3>(k_.data(), N1 * N0 * n2, single, outs, count_, static_cast<int>(N0),
+                             static_cast<int>(N1), static_cast<int>(n2), static_cast<int>(N2),
+                             st[0], st[1], st[2], fs[0], fs[1], fs[2], order_fft, s);
+    }
+  }
+
+  // Type 2: modes (caller's order and strides; `single` or the device table
+  // `ins`) -> deconvolved, zero-padded and transformed fine grids.
+  void backward(const Cplx<T>* single, const Cplx<T>* const* ins, const IntType* st,
+                T* const* fs, int order_fft, Cplx<T>* grids, hipStream_t s) {
+    const long long n0 = n_[0], n1 = n_[1], n2 = n_[2], N0 = N_[0], N1 = N_[1], N2 = N_[2];
+    const long long gstride = n0 * n1 * n2;
+    if constexpr (DIM == 2) {
+      launch_modes_in<T, 2>(single, ins, count_, h_.data(), N0 * n1, static_cast<int>(N0), 1,
+                            static_cast<int>(n1), static_cast<int>(N1), st[0], 0, st[1], fs[0],
+                            fs[1], fs[1], order_fft, s);
+    } else {
+      launch_modes_in<T, 3>(single, ins, count_, k_.data(), N1 * N0 * n2, static_cast<int>(N0),
+                            static_cast<int>(N1), static_cast<int>(n2), static_cast<int>(N2),
+                            st[0], st[1], st[2], fs[0], fs[1], fs[2], order_fft, s);
+      pz_.execute(k_.data());
+      launch_expand_transpose<T>(k_.data(), h_.data(), static_cast<int>(n2), static_cast<int>(n1),
+                                 static_cast<int>(N1), N0 * n2, N0 * n1,
+                                 static_cast<std::size_t>(count_ * N0), static_cast<int>(N0),
+                                 N1 * N0 * n2, n2, n2 * N0 * n1, n1, s);
+    }
+    py_.execute(h_.data());
+    launch_expand_transpose<T>(h_.data(), grids, static_cast<int>(n1), static_cast<int>(n0),
+                               static_cast<int>(N0), n1, n0,
+                               static_cast<std::size_t>(count_ * n2), static_cast<int>(n2),
+                               n2 * N0 * n1, N0 * n1, gstride, n1 * n0, s);
+    px_.execute(grids);
+  }
+
+private:
+  long long n_[3] = {1, 1, 1}, N_[3] = {1, 1, 1};
+  int count_ = 0;
+  FftPlan px_, py_, pz_;
+  DeviceBuffer<Cplx<T>> h_, k_;
+};
+
+}  // namespace nnf
