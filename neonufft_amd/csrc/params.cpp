@@ -165,6 +165,30 @@ BinGeom choose_bins(int dim, const IntType* n, int ns, std::size_t elem_bytes,
     if (pref[dmax] <= 1) break;
     pref[dmax] = std::max(1, pref[dmax] / 2);
   }
+  // Prefer bin edges that leave no partial last bin narrower than ns (the tile
+  // reduction then only needs a bin's direct neighbours): search the nearest
+  // edge within max(a quarter of the preferred one, ns) whose tile still fits.
+  for (int d = 0; d < dim; ++d) {
+    const int nd = static_cast<int>(n[d]);
+    const int B = std::min(pref[d], nd);
+    const auto good = [&](int b) { const int r = nd % b; return r == 0 || r >= ns; };
+    if (B < ns || good(B)) continue;
+    const int span = std::max(B / 4, ns);
+    for (int delta = 1; delta <= span; ++delta) {
+      bool found = false;
+      for (int cand : {B - delta, B + delta}) {
+        if (cand < ns || cand > nd) continue;
+        int trial[3] = {pref[0], pref[1], pref[2]};
+        trial[d] = cand;
+        if (good(cand) && bytes(trial) <= budget) {
+          pref[d] = cand;
+          found = true;
+          break;
+        }
+      }
+      if (found) break;
+    }
+  }
   for (int d = 0; d < 3; ++d) {
     if (d < dim) {
       const int nd = static_cast<int>(n[d]);

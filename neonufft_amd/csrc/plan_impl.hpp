@@ -289,10 +289,15 @@ public:
     for (int d = 1; d < DIM; ++d) strides[d] = strides[d - 1] * gi_.spread_grid_size[d - 1];
     for (IntType b = 0; b < batch_; ++b) {
       // The spread grid is read as CMCL-ordered modes (gpu/plan_t3_impl.hpp:227).
-      launch_upsample<T, DIM>(0, spread_grid_.data() + b * spread_cells_, strides,
-                              gi_.spread_grid_size, fs, fft_grid_.data(), gi_.fft_grid_size,
-                              stream_);
-      fft_.execute(fft_grid_.data());
+      if constexpr (DIM == 1) {
+        launch_upsample<T, DIM>(0, spread_grid_.data() + b * spread_cells_, strides,
+                                gi_.spread_grid_size, fs, fft_grid_.data(), gi_.fft_grid_size,
+                                stream_);
+        fft_.execute(fft_grid_.data());
+      } else {
+        pfft_.backward(spread_grid_.data() + b * spread_cells_, nullptr, strides, fs, 0,
+                       fft_grid_.data(), stream_);
+      }
       Cplx<T>* dst;
       if constexpr (kMixed) {
         dst = io_out_.data();
@@ -391,7 +396,11 @@ private:
     spread_grid_.reset(static_cast<std::size_t>(spread_cells_ * batch_), alloc_);
     launch_fill_zero<T>(spread_grid_.data(), spread_grid_.size(), stream_);
     fft_grid_.reset(fft_cells, alloc_);
-    fft_ = FftPlan(std::is_same_v<T, double>, DIM, gi_.fft_grid_size, sign_, 1, stream_, alloc_);
+    if constexpr (DIM == 1) {
+      fft_ = FftPlan(std::is_same_v<T, double>, DIM, gi_.fft_grid_size, sign_, 1, stream_, alloc_);
+    } else {
+      pfft_.init(gi_.fft_grid_size, gi_.spread_grid_size, sign_, 1, stream_, alloc_);
+    }
     for (int d = 0; d < DIM; ++d) {
       fs_[d].reset(static_cast<std::size_t>(gi_.fft_grid_size[d] / 2 + 1), alloc_);
       launch_fseries<T>(fs_[d].data(), gi_.fft_grid_size[d], ks_.quad, stream_);
@@ -417,7 +426,8 @@ private:
   DeviceBuffer<Cplx<T>> tiles_;           // spreading tiles of the input points
   CoverBuffers cover_in_;
   bool has_pre_ = false;
-  FftPlan fft_;
+  FftPlan fft_;                            // 1D plans
+  PrunedFft<T, DIM == 1 ? 2 : DIM> pfft_;  // 2D / 3D plans
   DeviceBuffer<T> fs_[3];
   PointSet<T, DIM> in_pts_, out_pts_;
 };
