@@ -50,13 +50,15 @@ def h2d(ptr, a):
 
 
 CASES = [
-    # (dim, modes, points, regular tile reduction expected)
-    (3, (32, 32, 32), 3000, True),
-    (2, (128, 100), 5000, True),
-    (1, (700,), 2000, True),
-    (3, (10, 12, 9), 500, False),
-    (2, (23, 2), 100, False),
-    (1, (100,), 300, False),
+    # (dim, modes, points, bin-owned (regular) tile reduction expected for
+    # f32 / f64; fp64 tiles are twice as large, so the LDS budget shrinks 3D
+    # bins below ns and the general reduction is used)
+    (3, (32, 32, 32), 3000, (True, False)),
+    (2, (128, 100), 5000, (True, True)),
+    (1, (700,), 2000, (True, True)),
+    (3, (10, 12, 9), 500, (False, False)),
+    (2, (23, 2), 100, (False, False)),
+    (1, (100,), 300, (False, False)),
 ]
 
 # fp32: Horner-fitted kernel (fit error ~1e-6 relative at ns = 6) and fp32
@@ -94,7 +96,7 @@ def _plan(prec, dim, modes, m, seed):
 def test_spread_and_accumulate(case, prec):
     dim, modes, m, regular = case
     rng, plan, info, u = _plan(prec, dim, modes, m, 7)
-    assert _regular(info) == regular
+    assert _regular(info) == regular[0 if prec == "f32" else 1]
     n, ns, beta = info["grid"], info["n_spread"], info["es_beta"]
     cells = int(np.prod(n))
     c1 = cplx(rng, m, prec)
