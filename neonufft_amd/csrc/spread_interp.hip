@@ -624,7 +624,7 @@ __device__ inline void reduce_bin(const ReduceJob<T, DIM>& job, int ns, uint32_t
   const std::size_t tc = tile_cells(g);
   const int cells = w[0] * w[1] * w[2];
   constexpr int kCombos = 1 << DIM;  // own tile or the one covering neighbour, per dimension
-  constexpr int kCells = 2;          // cells per thread per pass (loads in flight together)
+  constexpr int kCells = 1;  // cells per thread per pass (more passes beat fewer waves)
   for (int ibase = threadIdx.x; ibase < cells; ibase += kCells * kBinReduceThreads) {
     Cplx<T> v[kCells][kCombos];
     int tpos[kCells][kCombos], nidx[kCells][kCombos];
