@@ -151,7 +151,13 @@ BinGeom choose_bins(int dim, const IntType* n, int ns, std::size_t elem_bytes,
     for (int d = 0; d < dim; ++d)
       if (bin_override[d] > 0) pref[d] = bin_override[d];
   }
-  const std::size_t budget = 60 * 1024;  // LDS tile bytes per workgroup
+  // LDS tile bytes per workgroup (160 KB per CU on gfx950). Measured on MI355X
+  // with 128^3 modes: fp64 tiles need most of it, since at ns >= 9 a 60 KB
+  // tile leaves 4x4x4 bins whose halo is 50x the bin (tol 1e-10: type 1
+  // 39.6 -> 10.5 ms, type 2 20.3 -> 6.6 ms); fp32 tiles up to 100 KB keep
+  // 32x8x8 bins through ns = 8 (type 2 at ns = 7: 1.13 -> 0.97 ms, ns <= 6
+  // unchanged).
+  const std::size_t budget = elem_bytes >= 16 ? 144 * 1024 : 100 * 1024;
   auto bytes = [&](const int* b) {
     std::size_t cells = 1;
     for (int d = 0; d < dim; ++d) cells *= static_cast<std::size_t>(b[d] + ns);
