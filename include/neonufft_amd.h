@@ -132,11 +132,27 @@ int nnf_plan_t3_destroy(nnf_plan_t3 plan);
 
 /* ---- Tuning extension (no reference counterpart) -------------------------
  * The subgrid tile (bin) edge per dimension and the maximum number of points
- * one workgroup handles. Zero entries keep the built-in defaults. */
+ * one workgroup handles. Zero entries keep the built-in defaults.
+ * spread_mode selects the type-1 spreading of 2D plans:
+ *   NNF_SPREAD_TILES: per-subproblem tiles + tile reduction;
+ *   NNF_SPREAD_BANDS: one workgroup per band of whole fine-grid rows writes
+ *     its cells once; when n0 is a power of two >= 32, type 1 also runs the
+ *     x-direction FFT of the band in LDS and writes only the retained modes;
+ *   NNF_SPREAD_BANDS_GRID: row bands that always write the fine grid;
+ *   NNF_SPREAD_AUTO: NNF_SPREAD_BANDS whenever the grid suits the row view
+ *     (n0 + ns/2 and n1 at most 65535), else tiles.
+ * band_rows: fine-grid rows per band (0 = default). Other dimensions always
+ * use tiles. */
+#define NNF_SPREAD_AUTO 0
+#define NNF_SPREAD_TILES 1
+#define NNF_SPREAD_BANDS 2
+#define NNF_SPREAD_BANDS_GRID 3
 typedef struct nnf_tuning {
   int32_t bin_size[3];
   int32_t max_points;
-  int32_t reserved[4];
+  int32_t spread_mode;
+  int32_t band_rows;
+  int32_t reserved[2];
 } nnf_tuning;
 
 /* nnf_plan_create with explicit tuning (NULL = defaults). */
@@ -178,6 +194,8 @@ typedef struct nnf_plan_info {
   int64_t num_out_points;/* type 3 targets */
   int horner_degree;     /* fp32 ES polynomial degree (0 = exact exp/sqrt) */
   int quad_nodes;        /* Gauss-Legendre nodes on (0,1) for phihat */
+  int spread_mode;       /* type-1 spreading: NNF_SPREAD_TILES, _BANDS (fused x-FFT) or _BANDS_GRID */
+  int band_rows;         /* fine-grid rows per band (row bands), else 0 */
 } nnf_plan_info;
 
 int nnf_plan_get_info(nnf_plan plan, nnf_plan_info* info);
@@ -218,6 +236,12 @@ int nnf_plan_fseries(nnf_plan plan, int d, const void** fs);
 int nnf_batch_create(nnf_batch* batch, int precision, int dim, const nnf_options* opt, int sign,
                      int64_t count, const int64_t* num_nu, const void* const* loc,
                      const int64_t* modes, void* stream, int use_graph);
+/* nnf_batch_create with explicit tuning (NULL = defaults; bin_size,
+ * max_points, spread_mode and band_rows apply to every transform). */
+int nnf_batch_create_tuned(nnf_batch* batch, int precision, int dim, const nnf_options* opt,
+                           int sign, int64_t count, const int64_t* num_nu,
+                           const void* const* loc, const int64_t* modes, void* stream,
+                           int use_graph, const nnf_tuning* tuning);
 int nnf_batch_set_points(nnf_batch batch, const int64_t* num_nu, const void* const* loc);
 int nnf_batch_transform_type_1(nnf_batch batch, const void* const* in, void* const* out);
 int nnf_batch_transform_type_2(nnf_batch batch, const void* const* in, void* const* out);
@@ -232,6 +256,7 @@ typedef struct nnf_batch_info {
   int64_t num_shapes;       /* distinct mode shapes = batched FFT plans */
   int64_t grid_cells;       /* fine-grid cells over all transforms */
   int graph;                /* 1 once a captured graph is being replayed */
+  int spread_mode;          /* type-1 spreading: NNF_SPREAD_TILES, _BANDS or _BANDS_GRID */
 } nnf_batch_info;
 
 int nnf_batch_get_info(nnf_batch batch, nnf_batch_info* info);

@@ -183,9 +183,12 @@ class Plan:
 
     def __init__(self, opt: Options, sign: int, num_nu: int, loc: Sequence[Pointer],
                  modes: Sequence[int], stream=None, dtype=None, bin_size=None,
-                 max_points: int = 0):
+                 max_points: int = 0, spread_mode: str = "auto", band_rows: int = 0):
         """bin_size / max_points: optional tuning (extension; None/0 = defaults).
-        bin_size="auto" runs autotune() (both kernels) on these points first."""
+        bin_size="auto" runs autotune() (both kernels) on these points first.
+        spread_mode ("auto" | "tiles" | "bands" | "bands_grid") and band_rows
+        select the type-1 spreading of 2D plans (row bands with the fused
+        x-FFT by default where the grid suits them)."""
         self._L = _L.lib()
         if isinstance(bin_size, str):
             if bin_size != "auto":
@@ -198,13 +201,7 @@ class Plan:
         self.num_nu = int(num_nu)
         self._h = ctypes.c_void_p()
         vpp, _keep = _vpp([_ptr(p) for p in loc])
-        tune = None
-        if bin_size is not None or max_points:
-            tune = _L.NnfTuning()
-            for d, b in enumerate(list(bin_size or [])[:3]):
-                tune.bin_size[d] = int(b)
-            tune.max_points = int(max_points)
-            tune = ctypes.byref(tune)
+        tune = _tuning(bin_size, max_points, spread_mode, band_rows)
         _check(self._L.nnf_plan_create_tuned(ctypes.byref(self._h), self.precision, self.dim,
                                              ctypes.byref(opt._c()), int(sign), self.num_nu,
                                              vpp, _i64p(self.modes),
@@ -376,7 +373,8 @@ class BatchPlan:
 
     def __init__(self, opt: Options, sign: int, num_nu: Sequence[int],
                  loc: Sequence[Sequence[Pointer]], modes: Sequence[Sequence[int]],
-                 stream=None, dtype=None, use_graph: bool = True):
+                 stream=None, dtype=None, use_graph: bool = True, spread_mode: str = "auto",
+                 band_rows: int = 0):
         self._L = _L.lib()
         self.count = len(modes)
         if self.count == 0 or len(num_nu) != self.count or len(loc) != self.count:
@@ -392,10 +390,11 @@ class BatchPlan:
         vpp, _keep = _vpp(flat_loc)
         nn = (ctypes.c_int64 * self.count)(*self.num_nu)
         md = (ctypes.c_int64 * (self.count * self.dim))(*[v for m in self.modes for v in m])
-        _check(self._L.nnf_batch_create(ctypes.byref(self._h), self.precision, self.dim,
-                                        ctypes.byref(opt._c()), int(sign), self.count,
-                                        nn, vpp, md, ctypes.c_void_p(_stream(stream)),
-                                        int(bool(use_graph))))
+        _check(self._L.nnf_batch_create_tuned(ctypes.byref(self._h), self.precision, self.dim,
+                                              ctypes.byref(opt._c()), int(sign), self.count,
+                                              nn, vpp, md, ctypes.c_void_p(_stream(stream)),
+                                              int(bool(use_graph)),
+                                              _tuning(None, 0, spread_mode, band_rows)))
 
     def set_points(self, num_nu: Sequence[int], loc: Sequence[Sequence[Pointer]]) -> None:
         if len(num_nu) != self.count or len(loc) != self.count:
@@ -428,6 +427,7 @@ class BatchPlan:
             "dim": i.dim, "n_spread": i.n_spread, "count": i.count,
             "num_points": i.num_points, "num_subproblems": i.num_subproblems,
             "num_shapes": i.num_shapes, "grid_cells": i.grid_cells, "graph": bool(i.graph),
+            "spread_mode": _SPREAD_NAMES.get(i.spread_mode, "tiles"),
         }
 
     def close(self) -> None:
@@ -440,6 +440,25 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+_SPREAD_MODES = {"auto": 0, "tiles": 1, "bands": 2, "bands_grid": 3}
+_SPREAD_NAMES = {1: "tiles", 2: "bands", 3: "bands_grid"}
+
+
+def _tuning(bin_size, max_points, spread_mode, band_rows):
+    """nnf_tuning by reference, or None when everything is default."""
+    if spread_mode not in _SPREAD_MODES:
+        raise InputError(f"spread_mode must be one of {sorted(_SPREAD_MODES)}, not {spread_mode!r}")
+    if bin_size is None and not max_points and spread_mode == "auto" and not band_rows:
+        return None
+    tune = _L.NnfTuning()
+    for d, b in enumerate(list(bin_size or [])[:3]):
+        tune.bin_size[d] = int(b)
+    tune.max_points = int(max_points)
+    tune.spread_mode = _SPREAD_MODES[spread_mode]
+    tune.band_rows = int(band_rows)
+    return ctypes.byref(tune)
 
 
 def _info_dict(i: _L.NnfPlanInfo) -> dict:
@@ -458,4 +477,6 @@ def _info_dict(i: _L.NnfPlanInfo) -> dict:
         "num_out_points": i.num_out_points,
         "horner_degree": i.horner_degree,
         "quad_nodes": i.quad_nodes,
+        "spread_mode": _SPREAD_NAMES.get(i.spread_mode, "tiles"),
+        "band_rows": i.band_rows,
     }

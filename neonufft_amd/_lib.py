@@ -20,6 +20,7 @@ EXPORTED = (
     "nnf_batch_create", "nnf_batch_set_points", "nnf_batch_transform_type_1",
     "nnf_batch_transform_type_2", "nnf_batch_get_info", "nnf_batch_destroy",
     "nnf_autotune", "nnf_autotune_clear_cache", "nnf_plan_spread_tiles", "nnf_plan_reduce_tiles",
+    "nnf_batch_create_tuned",
 )
 
 
@@ -41,7 +42,9 @@ class NnfTuning(ctypes.Structure):
     _fields_ = [
         ("bin_size", ctypes.c_int32 * 3),
         ("max_points", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("spread_mode", ctypes.c_int32),
+        ("band_rows", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -61,6 +64,8 @@ class NnfPlanInfo(ctypes.Structure):
         ("num_out_points", ctypes.c_int64),
         ("horner_degree", ctypes.c_int),
         ("quad_nodes", ctypes.c_int),
+        ("spread_mode", ctypes.c_int),
+        ("band_rows", ctypes.c_int),
     ]
 
 
@@ -75,6 +80,7 @@ class NnfBatchInfo(ctypes.Structure):
         ("num_shapes", ctypes.c_int64),
         ("grid_cells", ctypes.c_int64),
         ("graph", ctypes.c_int),
+        ("spread_mode", ctypes.c_int),
     ]
 
 
@@ -131,6 +137,8 @@ def lib():
     L.nnf_plan_fseries.argtypes = [vp, c_int, vpp]
     L.nnf_batch_create.argtypes = [vpp, c_int, c_int, optp, c_int, i64, i64p, vpp, i64p, vp,
                                    c_int]
+    L.nnf_batch_create_tuned.argtypes = [vpp, c_int, c_int, optp, c_int, i64, i64p, vpp, i64p,
+                                         vp, c_int, vp]
     L.nnf_batch_set_points.argtypes = [vp, i64p, vpp]
     L.nnf_batch_transform_type_1.argtypes = [vp, vpp, vpp]
     L.nnf_batch_transform_type_2.argtypes = [vp, vpp, vpp]

@@ -38,6 +38,7 @@ DEVICE = [
 SOURCES = [
     ("points.hip", True),
     ("spread_interp.hip", True),
+    ("band.hip", True),
     ("grid.hip", True),
     ("pruned.hip", True),
     ("params.cpp", False),

@@ -1,0 +1,664 @@
+// Row-band spreading for 2D type 1.
+//
+// Same sums as the reference's spread_2d_kernel
+// (src/neonufft/gpu/kernels/spreading_kernel.cu:477-560): cell
+// ((i0x + a) mod n0, (i0y + b) mod n1) receives phi_x(a) phi_y(b) c for every
+// point. The decomposition is MI355X-specific: one workgroup owns a band of
+// whole fine-grid rows, accumulates every point whose kernel rows reach the
+// band in LDS (fixed point, integer atomics) and writes each of its cells
+// once. No tiles are stored, there is no tile reduction and no zero fill, and
+// the band of rows is the natural unit for a fused x-direction FFT.
+//
+// Points are read in a row-sorted view (sorted by the first kernel row i0y),
+// so the points of a band are one or two contiguous ranges; their strengths
+// are first gathered into that order so that the bands read them coalesced.
+#include "es_device.hpp"
+
+namespace nnf {
+
+namespace {
+
+constexpr int kRowBlock = 256;
+constexpr int kGatherItems = 8;
+constexpr int kBoundBlocks = 64;
+// Band workgroup size and points per thread per pass: every thread issues the
+// loads of kBandBatch points before using them, so 16 waves per CU (fp32) keep
+// several loads in flight each while the LDS atomics of earlier points run.
+template <typename T>
+constexpr int kBandThreads = sizeof(T) == 4 ? 1024 : 512;
+constexpr int kBandBatch = 4;
+
+// First kernel cell (i0x, i0y) of a tile-sorted 2D record, from its bin and
+// its bin-relative cell (the packing of bin_scatter_kernel).
+template <typename T>
+__device__ inline void rec_start(const PointRec<T, 2>& r, const BinGeom& g, int ns, int& i0x,
+                                 int& i0y) {
+  const int bx = static_cast<int>(r.bin % static_cast<uint32_t>(g.nbins[0]));
+  const int by = static_cast<int>(r.bin / static_cast<uint32_t>(g.nbins[0]));
+  i0x = bx * g.bin[0] - ns / 2 + static_cast<int>(r.cell & CellPack<2>::mask);
+  i0y = by * g.bin[1] - ns / 2 + static_cast<int>(r.cell >> CellPack<2>::bits);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock)
+    row_count_kernel(const PointRec<T, 2>* __restrict__ recs, std::size_t m, BinGeom g, int ns,
+                     uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
+  for (std::size_t p = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kRowBlock; p < m;
+       p += static_cast<std::size_t>(gridDim.x) * kRowBlock) {
+    int i0x, i0y;
+    rec_start(recs[p], g, ns, i0x, i0y);
+    // i0y = ceil(y - ns/2) lies in [-ns/2, n1 - 1]
+    const int row = i0y < 0 ? i0y + g.n[1] : i0y;
+    rank[p] = atomicAdd(&count[row], 1u);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock)
+    row_scatter_kernel(const PointRec<T, 2>* __restrict__ recs, const uint32_t* __restrict__ perm,
+                       std::size_t m, BinGeom g, int ns, const uint32_t* __restrict__ row_start,
+                       const uint32_t* __restrict__ rank, PointRec<T, 2>* __restrict__ out,
+                       uint32_t* __restrict__ perm_out) {
+  for (std::size_t p = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kRowBlock; p < m;
+       p += static_cast<std::size_t>(gridDim.x) * kRowBlock) {
+    PointRec<T, 2> r = recs[p];
+    int i0x, i0y;
+    rec_start(r, g, ns, i0x, i0y);
+    const int row = i0y < 0 ? i0y + g.n[1] : i0y;
+    r.cell = static_cast<uint32_t>(i0x + ns / 2) | (static_cast<uint32_t>(row) << CellPack<2>::bits);
+    r.bin = static_cast<uint32_t>(row);
+    const uint32_t pos = row_start[row] + rank[p];
+    out[pos] = r;
+    perm_out[pos] = perm[p];
+  }
+}
+
+// Index of the job owning flattened item `i` (jobs sorted by `off`).
+template <typename T>
+__device__ inline int find_job(const BandJob<T>* jobs, int njobs, std::size_t i, bool by_band) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const std::size_t o = by_band ? jobs[mid].band_off : jobs[mid].pt_off;
+    if (o <= i) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return lo;
+}
+
+// sorted[p] = in[perm[p]] over the points of every job (flattened).
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock)
+    band_gather_kernel(const BandJob<T>* __restrict__ jobs, BandJob<T> single, int njobs,
+                       std::size_t total) {
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * (kRowBlock * kGatherItems);
+  if (base >= total) return;
+  int j = jobs ? find_job(jobs, njobs, base, false) : 0;
+  BandJob<T> job = jobs ? jobs[j] : single;
+  // all index loads, then all strength loads, then the stores
+  Cplx<T>* dst[kGatherItems];
+  const Cplx<T>* src[kGatherItems];
+#pragma unroll
+  for (int k = 0; k < kGatherItems; ++k) {
+    const std::size_t i = base + threadIdx.x + static_cast<std::size_t>(k) * kRowBlock;
+    dst[k] = nullptr;
+    src[k] = nullptr;
+    if (i < total) {
+      while (i >= job.pt_off + job.m) job = jobs[++j];
+      const std::size_t p = i - job.pt_off;
+      dst[k] = job.sorted + p;
+      src[k] = job.in + job.perm[p];
+    }
+  }
+  Cplx<T> v[kGatherItems];
+#pragma unroll
+  for (int k = 0; k < kGatherItems; ++k) v[k] = src[k] ? *src[k] : Cplx<T>{T(0), T(0)};
+#pragma unroll
+  for (int k = 0; k < kGatherItems; ++k) {
+    if (dst[k]) *dst[k] = v[k];
+  }
+}
+
+template <typename T>
+__device__ inline T wave_max(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// x-direction FFT of the band in LDS (fused type 1), decimation in frequency:
+// the rows are accumulated in natural x order (bank-ordered points stay
+// conflict-free) and the transform leaves X[f] at position bitrev(f):
+//   stage B: half-spans n0/2 .. 32 as radix-4 passes through LDS (consecutive
+//            lanes take consecutive positions, conflict-free), plus one
+//            radix-2 pass at half-span 32 when their count is odd; a thread
+//            keeps one position across the rows of the band, so it loads its
+//            twiddles once per pass and overlaps the loads of two rows;
+//   stage A: half-spans 16 .. 1 inside aligned 32-cell blocks: a quad of
+//            lanes owns a block, 8 consecutive positions per lane (16 and 8
+//            by DPP quad permutations, 4, 2, 1 in registers).
+// The first stage that runs converts the fixed-point sums to T in place.
+// Twiddles are stored level by level: W_N^q = exp(sign 2 pi i q / N) at
+// tw[N/2 + q] (q < N/2, N = 2, 4, ... up to the largest n0), so the twiddles
+// of consecutive positions are consecutive in memory.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline Cplx<T> twid(const Cplx<T>* __restrict__ tw, int n, int q) {
+  return tw[(n >> 1) + q];
+}
+
+// (a, b) <- (a + b, (a - b) w)
+template <typename T>
+__device__ inline void dif(T& ar, T& ai, T& br, T& bi, const Cplx<T>& w) {
+  const T dr = ar - br, di = ai - bi;
+  ar += br;
+  ai += bi;
+  br = dr * w.x - di * w.y;
+  bi = dr * w.y + di * w.x;
+}
+
+// The value of `v` in lane ^ MASK (MASK 1 or 2; quad permutation, no LDS).
+template <int MASK>
+__device__ inline int quad_xor_bits(int v) {
+  constexpr int ctrl = MASK == 1 ? 0xB1 : 0x4E;  // quad_perm [1,0,3,2] / [2,3,0,1]
+  return __builtin_amdgcn_update_dpp(0, v, ctrl, 0xF, 0xF, false);
+}
+template <int MASK>
+__device__ inline float quad_xor(float v) {
+  return __int_as_float(quad_xor_bits<MASK>(__float_as_int(v)));
+}
+template <int MASK>
+__device__ inline double quad_xor(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = quad_xor_bits<MASK>(static_cast<int>(b));
+  const int hi = quad_xor_bits<MASK>(static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned int>(lo)));
+}
+
+// Level of half-span 16 (MASK 2) or 8 (MASK 1) across the lanes of a quad:
+// the lane whose chunk bit is clear holds the top element a, its partner b;
+// top <- a + b, bottom <- (a - b) w.
+template <int MASK, typename T>
+__device__ inline void dif_quad(T& xr, T& xi, bool top, const Cplx<T>& w) {
+  const T pr = quad_xor<MASK>(xr), pi = quad_xor<MASK>(xi);
+  const T dr = pr - xr, di = pi - xi;  // bottom: a - b with a = partner
+  xr = top ? xr + pr : dr * w.x - di * w.y;
+  xi = top ? xi + pi : dr * w.y + di * w.x;
+}
+
+// Element i of a row plane as T: converted from fixed point in the first
+// stage, else read back as T.
+template <bool CONV, typename T, typename Acc>
+__device__ inline T ldv(const Acc* ip, const T* fp, int i, T inv) {
+  if constexpr (CONV) {
+    return static_cast<T>(ip[i]) * inv;
+  } else {
+    return fp[i];
+  }
+}
+
+// Radix-4 DIF over the levels 2u then u at positions base + {0, u, 2u, 3u}.
+template <bool CONV, typename T, typename Acc>
+__device__ inline void dif4(const Acc* ire, const Acc* iim, T* fre, T* fim, int base, int u,
+                            const Cplx<T>& w40, const Cplx<T>& w41, const Cplx<T>& w2, T inv) {
+  T x0r = ldv<CONV>(ire, fre, base, inv), x0i = ldv<CONV>(iim, fim, base, inv);
+  T x1r = ldv<CONV>(ire, fre, base + u, inv), x1i = ldv<CONV>(iim, fim, base + u, inv);
+  T x2r = ldv<CONV>(ire, fre, base + 2 * u, inv), x2i = ldv<CONV>(iim, fim, base + 2 * u, inv);
+  T x3r = ldv<CONV>(ire, fre, base + 3 * u, inv), x3i = ldv<CONV>(iim, fim, base + 3 * u, inv);
+  dif(x0r, x0i, x2r, x2i, w40);
+  dif(x1r, x1i, x3r, x3i, w41);
+  dif(x0r, x0i, x1r, x1i, w2);
+  dif(x2r, x2i, x3r, x3i, w2);
+  fre[base] = x0r;
+  fim[base] = x0i;
+  fre[base + u] = x1r;
+  fim[base + u] = x1i;
+  fre[base + 2 * u] = x2r;
+  fim[base + 2 * u] = x2i;
+  fre[base + 3 * u] = x3r;
+  fim[base + 3 * u] = x3i;
+}
+
+template <bool CONV, typename T, int NT, typename Acc>
+__device__ inline void dif4_pass(const Acc* ire, const Acc* iim, T* fre, T* fim, int rk, int pitch,
+                                 int log2n0, int u, T inv, const Cplx<T>* __restrict__ tw) {
+  const int lq = log2n0 - 2;  // log2 items per row
+  const int items = 1 << lq;
+  const int rstep = items >= NT ? 1 : NT >> lq;
+  for (int j0 = 0; j0 < items; j0 += NT) {
+    const int j = j0 + (items >= NT ? static_cast<int>(threadIdx.x)
+                                    : static_cast<int>(threadIdx.x) & (items - 1));
+    const int r0 = items >= NT ? 0 : static_cast<int>(threadIdx.x) >> lq;
+    const int q = j & (u - 1);
+    const int p = (j - q) * 4 + q;
+    const Cplx<T> w40 = twid(tw, 4 * u, q), w41 = twid(tw, 4 * u, q + u);
+    const Cplx<T> w2 = twid(tw, 2 * u, q);
+    int r = r0;
+    for (; r + rstep < rk; r += 2 * rstep) {
+      dif4<CONV>(ire, iim, fre, fim, r * pitch + p, u, w40, w41, w2, inv);
+      dif4<CONV>(ire, iim, fre, fim, (r + rstep) * pitch + p, u, w40, w41, w2, inv);
+    }
+    if (r < rk) dif4<CONV>(ire, iim, fre, fim, r * pitch + p, u, w40, w41, w2, inv);
+  }
+}
+
+template <bool CONV, typename T, int NT, typename Acc>
+__device__ inline void dif2_pass32(const Acc* ire, const Acc* iim, T* fre, T* fim, int rk,
+                                   int pitch, int log2n0, T inv, const Cplx<T>* __restrict__ tw) {
+  const int lh = log2n0 - 1;
+  const int items = 1 << lh;
+  const int rstep = items >= NT ? 1 : NT >> lh;
+  for (int j0 = 0; j0 < items; j0 += NT) {
+    const int j = j0 + (items >= NT ? static_cast<int>(threadIdx.x)
+                                    : static_cast<int>(threadIdx.x) & (items - 1));
+    const int r0 = items >= NT ? 0 : static_cast<int>(threadIdx.x) >> lh;
+    const int q = j & 31;
+    const int p = (j - q) * 2 + q;
+    const Cplx<T> w = twid(tw, 64, q);
+    for (int r = r0; r < rk; r += rstep) {
+      const int a = r * pitch + p;
+      T ar = ldv<CONV>(ire, fre, a, inv), ai = ldv<CONV>(iim, fim, a, inv);
+      T br = ldv<CONV>(ire, fre, a + 32, inv), bi = ldv<CONV>(iim, fim, a + 32, inv);
+      dif(ar, ai, br, bi, w);
+      fre[a] = ar;
+      fim[a] = ai;
+      fre[a + 32] = br;
+      fim[a + 32] = bi;
+    }
+  }
+}
+
+template <bool CONV, typename T, int NT, typename Acc>
+__device__ inline void dif_stage_a(const Acc* ire, const Acc* iim, T* fre, T* fim, int rk,
+                                   int pitch, int log2n0, T inv, const Cplx<T>* __restrict__ tw) {
+  const int lc = log2n0 - 3;  // log2 chunks of 8 per row
+  const int chunks = 1 << lc;
+  const int rstep = chunks >= NT ? 1 : NT >> lc;
+  const int c = threadIdx.x & 3;
+  for (int k0 = 0; k0 < chunks; k0 += NT) {
+    const int k = k0 + (chunks >= NT ? static_cast<int>(threadIdx.x)
+                                     : static_cast<int>(threadIdx.x) & (chunks - 1));
+    const int r0 = chunks >= NT ? 0 : static_cast<int>(threadIdx.x) >> lc;
+    for (int r = r0; r < rk; r += rstep) {
+      const int base = r * pitch + 8 * k;
+      T xr[8], xi[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        xr[e] = ldv<CONV>(ire, fre, base + e, inv);
+        xi[e] = ldv<CONV>(iim, fim, base + e, inv);
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        dif_quad<2>(xr[e], xi[e], (c & 2) == 0, twid(tw, 32, 8 * (c & 1) + e));
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) dif_quad<1>(xr[e], xi[e], (c & 1) == 0, twid(tw, 16, e));
+#pragma unroll
+      for (int e = 0; e < 4; ++e) dif(xr[e], xi[e], xr[e + 4], xi[e + 4], twid(tw, 8, e));
+#pragma unroll
+      for (int e = 0; e < 8; e += 4) {
+        dif(xr[e], xi[e], xr[e + 2], xi[e + 2], twid(tw, 4, 0));
+        dif(xr[e + 1], xi[e + 1], xr[e + 3], xi[e + 3], twid(tw, 4, 1));
+      }
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) dif(xr[e], xi[e], xr[e + 1], xi[e + 1], twid(tw, 2, 0));
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        fre[base + e] = xr[e];
+        fim[base + e] = xi[e];
+      }
+    }
+  }
+}
+
+template <typename T, int NT, typename Acc>
+__device__ inline void band_fft_x(Acc* ire, Acc* iim, int rk, int pitch, int log2n0, T inv,
+                                  const Cplx<T>* __restrict__ tw) {
+  T* fre = reinterpret_cast<T*>(ire);
+  T* fim = reinterpret_cast<T*>(iim);
+  const int lb = log2n0 - 5;  // levels of half-span >= 32
+  bool conv = true;
+  int u = 1 << (log2n0 - 2);
+  for (int k = 0; k < lb / 2; ++k, u >>= 2) {
+    if (conv) {
+      dif4_pass<true, T, NT>(ire, iim, fre, fim, rk, pitch, log2n0, u, inv, tw);
+    } else {
+      dif4_pass<false, T, NT>(ire, iim, fre, fim, rk, pitch, log2n0, u, inv, tw);
+    }
+    conv = false;
+    __syncthreads();
+  }
+  if (lb & 1) {
+    if (conv) {
+      dif2_pass32<true, T, NT>(ire, iim, fre, fim, rk, pitch, log2n0, inv, tw);
+    } else {
+      dif2_pass32<false, T, NT>(ire, iim, fre, fim, rk, pitch, log2n0, inv, tw);
+    }
+    conv = false;
+    __syncthreads();
+  }
+  if (conv) {
+    dif_stage_a<true, T, NT>(ire, iim, fre, fim, rk, pitch, log2n0, inv, tw);
+  } else {
+    dif_stage_a<false, T, NT>(ire, iim, fre, fim, rk, pitch, log2n0, inv, tw);
+  }
+  __syncthreads();
+}
+
+// One band of rows [y0, y0 + rk) of one job.
+//  1. bound: every cell value is at most (points whose kernel x-window meets
+//     the cell's 1/64 column block) x max |Re c| + |Im c|; the largest block
+//     count times the largest strength fixes the fixed-point scale;
+//  2. accumulation with integer LDS atomics (rows outside the band skipped;
+//     a row no lane of a wave needs is skipped by the whole wave);
+//  3. each cell converted back once and written (or added) to the grid, or
+//     (FFT) the rows transformed along x in LDS and only the retained modes
+//     written, transposed, to job.h[x' * n1 + y].
+template <typename T, int NS, bool FFT>
+__global__ void __launch_bounds__(kBandThreads<T>)
+    band_spread_kernel(const BandJob<T>* __restrict__ jobs, BandJob<T> single, int njobs,
+                       EsParams es, int accumulate) {
+  using Acc = typename FixedAcc<T>::type;
+  constexpr int NT = kBandThreads<T>;
+  constexpr int KB = kBandBatch;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ T wred[NT / 64];
+  __shared__ uint32_t blk[kBoundBlocks];
+
+  const BandJob<T> job = jobs ? jobs[find_job(jobs, njobs, blockIdx.x, true)] : single;
+  const int n0 = job.n0, n1 = job.n1, pitch = job.pitch;
+  const int y0 = static_cast<int>(blockIdx.x - job.band_off) * job.rows;
+  const int rk = min(job.rows, n1 - y0);
+  Acc* tre = reinterpret_cast<Acc*>(smem_raw);
+  Acc* tim = tre + rk * pitch;
+  for (int i = threadIdx.x; i < 2 * rk * pitch; i += NT) tre[i] = Acc(0);
+  if (threadIdx.x < kBoundBlocks) blk[threadIdx.x] = 0u;
+
+  // Points whose first kernel row lies in [y0 - NS + 1, y0 + rk - 1] (mod
+  // n1): one range of the row-sorted view, or two when it wraps below row 0
+  // (rows of the wrapped range sit n1 below their stored row).
+  const int lo_row = y0 - (NS - 1);
+  uint32_t b0, e0, b1 = 0, e1 = 0;
+  int shift0 = 0;
+  if (lo_row < 0) {
+    b0 = job.row_start[lo_row + n1];
+    e0 = job.row_start[n1];
+    shift0 = -n1;
+    b1 = job.row_start[0];
+    e1 = job.row_start[y0 + rk];
+  } else {
+    b0 = job.row_start[lo_row];
+    e0 = job.row_start[y0 + rk];
+  }
+  const uint32_t cnt0 = e0 - b0;
+  const uint32_t total = cnt0 + (e1 - b1);
+  const PointRec<T, 2>* __restrict__ pts = job.pts;
+  const Cplx<T>* __restrict__ str = job.sorted;
+
+  const int bw = max(1, (n0 + kBoundBlocks - 1) / kBoundBlocks);
+  const int nb = (n0 + bw - 1) / bw;
+  __syncthreads();  // zeroed counters before the bound pass
+
+  T cm = T(0);
+  for (uint32_t q0 = 0; q0 < total; q0 += NT * KB) {
+    uint32_t cell[KB];
+    Cplx<T> c[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const uint32_t q = q0 + threadIdx.x + k * NT;
+      cell[k] = 0xffffffffu;
+      c[k] = Cplx<T>{T(0), T(0)};
+      if (q < total) {
+        const uint32_t p = q < cnt0 ? b0 + q : b1 + (q - cnt0);
+        cell[k] = pts[p].cell;
+        c[k] = str[p];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      if (cell[k] == 0xffffffffu) continue;
+      cm = max(cm, fabs(c[k].x) + fabs(c[k].y));
+      const int x0 = static_cast<int>(cell[k] & CellPack<2>::mask) - NS / 2;
+      const int xa = x0 < 0 ? x0 + n0 : x0;
+      const int xe = x0 + NS - 1;
+      const int xb = xe >= n0 ? xe - n0 : xe;
+      const int kb = xb / bw;
+      for (int b = xa / bw;; b = (b + 1 == nb ? 0 : b + 1)) {
+        atomicAdd(&blk[b], 1u);
+        if (b == kb) break;
+      }
+    }
+  }
+  cm = wave_max(cm);
+  if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = cm;
+  __syncthreads();
+  T cmax = T(0);
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) cmax = max(cmax, wred[w]);
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t bmax = wave_max(lane < static_cast<uint32_t>(nb) ? blk[lane] : 0u);
+  const T bound = static_cast<T>(bmax) * cmax;
+  const T one = static_cast<T>(1ull << FixedAcc<T>::bits);
+  const T scale = bound > T(0) ? one / bound : T(0);
+
+  for (uint32_t q0 = 0; q0 < total; q0 += NT * KB) {
+    PointRec<T, 2> rec[KB];
+    Cplx<T> c[KB];
+    int lr0[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const uint32_t q = q0 + threadIdx.x + k * NT;
+      lr0[k] = NS;  // no row of the band
+      rec[k] = PointRec<T, 2>{};
+      c[k] = Cplx<T>{T(0), T(0)};
+      if (q < total) {
+        const bool first = q < cnt0;
+        const uint32_t p = first ? b0 + q : b1 + (q - cnt0);
+        rec[k] = pts[p];
+        c[k] = str[p];
+        lr0[k] = first ? shift0 - y0 : -y0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const bool valid = lr0[k] != NS;
+      if (__ballot(valid) == 0) break;
+      const T cr = c[k].x * scale, ci = c[k].y * scale;
+      const int x0 = static_cast<int>(rec[k].cell & CellPack<2>::mask) - NS / 2;
+      const int l0 = valid ? static_cast<int>(rec[k].cell >> CellPack<2>::bits) + lr0[k] : rk;
+      T kx[NS];
+      es_weights<T, NS>(rec[k].x[0], es, kx);
+      int xs[NS];
+#pragma unroll
+      for (int a = 0; a < NS; ++a) {
+        xs[a] = wrap_once(x0 + a, n0);
+
+      }
+#pragma unroll
+      for (int b = 0; b < NS; ++b) {
+        const int r = l0 + b;
+        const bool act = static_cast<unsigned>(r) < static_cast<unsigned>(rk);
+        if (__ballot(act) == 0) continue;
+        if (act) {
+          const T wy = es_piece<NS>(rec[k].x[1], b, es);
+          const T vr = wy * cr, vi = wy * ci;
+          Acc* rr = tre + r * pitch;
+          Acc* ri = tim + r * pitch;
+#pragma unroll
+          for (int a = 0; a < NS; ++a) {
+            lds_add_fixed(&rr[xs[a]], to_fixed(kx[a] * vr));
+            lds_add_fixed(&ri[xs[a]], to_fixed(kx[a] * vi));
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  const T inv = bound > T(0) ? bound / one : T(0);
+  if constexpr (FFT) {
+    band_fft_x<T, NT>(tre, tim, rk, pitch, job.log2n0, inv, job.tw);
+    // retained mode xp of row r: X[f] at position bitrev(f). Lanes take
+    // (r fastest, then xp from eight interleaved strides of g modes) so that
+    // eight lanes write 8 rows of one mode contiguously and the bitrev
+    // positions of a wave spread over the LDS banks.
+    const T* fre = reinterpret_cast<const T*>(tre);
+    const T* fim = reinterpret_cast<const T*>(tim);
+    const int lr = job.log2rows;
+    const int N0 = job.N0;
+    const int P = N0 - N0 / 2;
+    const int g = (N0 + 7) >> 3;
+    const int total_out = (8 * g) << lr;
+    const int sh = 32 - job.log2n0;
+    Cplx<T>* __restrict__ h = job.h;
+    for (int i = threadIdx.x; i < total_out; i += NT) {
+      const int r = i & ((1 << lr) - 1);
+      const int j = i >> lr;
+      const int xp = (j & 7) * g + (j >> 3);
+      if (r >= rk || xp >= N0) continue;
+      const int f = xp < P ? xp : xp + n0 - N0;
+      const int pos = r * pitch + static_cast<int>(__brev(static_cast<unsigned>(f)) >> sh);
+      h[static_cast<std::size_t>(xp) * n1 + y0 + r] = Cplx<T>{fre[pos], fim[pos]};
+    }
+    return;
+  }
+
+  // Each cell converted back once; threads cover several rows at a time when
+  // a row is shorter than the workgroup.
+  Cplx<T>* __restrict__ grid = job.grid + static_cast<std::size_t>(y0) * n0;
+  const int rpp = n0 >= NT ? 1 : NT / n0;  // rows per pass
+  const int tr = n0 >= NT ? 0 : static_cast<int>(threadIdx.x) / n0;
+  const int tx = n0 >= NT ? static_cast<int>(threadIdx.x) : static_cast<int>(threadIdx.x) - tr * n0;
+  if (tr < rpp) {
+    for (int r = tr; r < rk; r += rpp) {
+      Cplx<T>* row = grid + static_cast<std::size_t>(r) * n0;
+      for (int x = tx; x < n0; x += (n0 >= NT ? NT : n0)) {
+        Cplx<T> v{static_cast<T>(tre[r * pitch + x]) * inv,
+                  static_cast<T>(tim[r * pitch + x]) * inv};
+        if (accumulate) {
+          const Cplx<T> o = row[x];
+          v.x += o.x;
+          v.y += o.y;
+        }
+        row[x] = v;
+      }
+    }
+  }
+}
+
+template <typename T, int NS>
+void band_spread_ns(const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
+                    std::size_t bands, std::size_t lds, bool accumulate, bool fft,
+                    const EsParams& es, hipStream_t stream) {
+  if (fft) {
+    hipLaunchKernelGGL((band_spread_kernel<T, NS, true>), dim3(static_cast<unsigned>(bands)),
+                       dim3(kBandThreads<T>), lds, stream, jobs, single, njobs, es, 0);
+  } else {
+    hipLaunchKernelGGL((band_spread_kernel<T, NS, false>), dim3(static_cast<unsigned>(bands)),
+                       dim3(kBandThreads<T>), lds, stream, jobs, single, njobs, es,
+                       accumulate ? 1 : 0);
+  }
+  check_launch("band_spread_kernel");
+}
+
+unsigned blocks_for(std::size_t work, int per_block) {
+  std::size_t g = (work + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > 65535u * 16u) g = 65535u * 16u;
+  return static_cast<unsigned>(g);
+}
+
+}  // namespace
+
+template <typename T>
+void launch_row_count(const PointRec<T, 2>* recs, std::size_t m, const BinGeom& g, int ns,
+                      uint32_t* count, uint32_t* rank, hipStream_t stream) {
+  if (m == 0) return;
+  hipLaunchKernelGGL((row_count_kernel<T>), dim3(blocks_for(m, kRowBlock)), dim3(kRowBlock), 0,
+                     stream, recs, m, g, ns, count, rank);
+  check_launch("row_count_kernel");
+}
+
+template <typename T>
+void launch_row_scatter(const PointRec<T, 2>* recs, const uint32_t* perm, std::size_t m,
+                        const BinGeom& g, int ns, const uint32_t* row_start, const uint32_t* rank,
+                        PointRec<T, 2>* out, uint32_t* perm_out, hipStream_t stream) {
+  if (m == 0) return;
+  hipLaunchKernelGGL((row_scatter_kernel<T>), dim3(blocks_for(m, kRowBlock)), dim3(kRowBlock), 0,
+                     stream, recs, perm, m, g, ns, row_start, rank, out, perm_out);
+  check_launch("row_scatter_kernel");
+}
+
+template <typename T>
+void launch_band_gather(const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
+                        std::size_t total_points, hipStream_t stream) {
+  if (total_points == 0) return;
+  const std::size_t per = static_cast<std::size_t>(kRowBlock) * kGatherItems;
+  const std::size_t blocks = (total_points + per - 1) / per;
+  if (blocks > 0x7fffffffu) throw neonufft::InputError("Neonufft:: too many points");
+  hipLaunchKernelGGL((band_gather_kernel<T>), dim3(static_cast<unsigned>(blocks)),
+                     dim3(kRowBlock), 0, stream, jobs, single, njobs, total_points);
+  check_launch("band_gather_kernel");
+}
+
+template <typename T>
+void launch_band_spread(int ns, const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
+                        std::size_t total_bands, std::size_t lds_bytes, bool accumulate, bool fft,
+                        const EsParams& es, hipStream_t stream) {
+  if (total_bands == 0) return;
+  if (fft && accumulate) throw neonufft::InternalError("fused band FFT cannot accumulate");
+  if (total_bands > 0x7fffffffu) throw neonufft::InputError("Neonufft:: too many row bands");
+  constexpr int DIM = 2;
+  (void)DIM;
+#define NNF_BAND_CASE(NSV)                                                                    \
+  case NSV:                                                                                  \
+    band_spread_ns<T, NSV>(jobs, single, njobs, total_bands, lds_bytes, accumulate, fft, es,   \
+                           stream);                                                          \
+    break;
+  switch (ns) {
+    NNF_BAND_CASE(2)
+    NNF_BAND_CASE(3)
+    NNF_BAND_CASE(4)
+    NNF_BAND_CASE(5)
+    NNF_BAND_CASE(6)
+    NNF_BAND_CASE(7)
+    NNF_BAND_CASE(8)
+    NNF_BAND_CASE(9)
+    NNF_BAND_CASE(10)
+    NNF_BAND_CASE(11)
+    NNF_BAND_CASE(12)
+    NNF_BAND_CASE(13)
+    NNF_BAND_CASE(14)
+    NNF_BAND_CASE(15)
+    NNF_BAND_CASE(16)
+    default:
+      throw neonufft::InternalError("n_spread not in [2, 16]");
+  }
+#undef NNF_BAND_CASE
+}
+
+std::size_t band_lds_bytes(int rows, int pitch, std::size_t acc_bytes) {
+  return 2 * static_cast<std::size_t>(rows) * static_cast<std::size_t>(pitch) * acc_bytes;
+}
+
+#define NNF_INST_BAND(T)                                                                        \
+  template void launch_row_count<T>(const PointRec<T, 2>*, std::size_t, const BinGeom&, int,    \
+                                    uint32_t*, uint32_t*, hipStream_t);                         \
+  template void launch_row_scatter<T>(const PointRec<T, 2>*, const uint32_t*, std::size_t,      \
+                                      const BinGeom&, int, const uint32_t*, const uint32_t*,    \
+                                      PointRec<T, 2>*, uint32_t*, hipStream_t);                 \
+  template void launch_band_gather<T>(const BandJob<T>*, const BandJob<T>&, int, std::size_t,   \
+                                      hipStream_t);                                             \
+  template void launch_band_spread<T>(int, const BandJob<T>*, const BandJob<T>&, int,           \
+                                      std::size_t, std::size_t, bool, bool, const EsParams&,    \
+                                      hipStream_t);
+
+NNF_INST_BAND(float)
+NNF_INST_BAND(double)
+
+}  // namespace nnf

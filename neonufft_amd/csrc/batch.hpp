@@ -132,6 +132,7 @@ public:
       g.offset = off;
       for (std::size_t k = 0; k < g.members.size(); ++k) {
         items_[g.members[k]].grid_off = off + k * g.cells;
+        items_[g.members[k]].member = static_cast<int>(k);
       }
       off += g.cells * g.members.size();
     }
@@ -155,7 +156,17 @@ public:
       max_bins_ = std::max<std::size_t>(
           max_bins_, static_cast<std::size_t>(g.geom.nbins[0]) * g.geom.nbins[1] * g.geom.nbins[2]);
       regular_ = regular_ && bins_regular(DIM, g.geom, ks_.ns);
+      if constexpr (DIM == 2) {
+        g.band = plan_band_layout(g.n, ks_.ns, sizeof(T), tune_, 1);
+        bands_ = bands_ && g.band.rows > 0;
+        fused_ = fused_ && g.band.fft;
+        band_lds_ = std::max(band_lds_, band_lds_bytes(g.band.rows, g.band.pitch, sizeof(T)));
+        tw_len_ = std::max<IntType>(tw_len_, g.n[0]);
+      }
     }
+    if constexpr (DIM != 2) bands_ = false;
+    fused_ = fused_ && bands_;
+    if (fused_) upload_twiddles<T>(tw_, tw_len_, sign_, stream_, alloc_);
     std::size_t io_off = 0;
     for (Group& g : groups_) {
       g.io_off = io_off;
@@ -171,6 +182,12 @@ public:
     host_ijobs_.resize(count_);
     host_io_.resize(count_);
     up_sjobs_.init(sizeof(SpreadJob<T, DIM>) * count_);
+    if (bands_) {
+      bjobs_.reset(count_, alloc_);
+      host_bjobs_.resize(count_);
+      rows_.resize(count_);
+      up_bjobs_.init(sizeof(BandJob<T>) * count_);
+    }
     up_ijobs_.init(sizeof(InterpJob<T, DIM>) * count_);
     up_io_.init(sizeof(void*) * count_);
     check_hip(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -200,11 +217,25 @@ public:
                     ks_.ns, tune_.max_points, stream_, alloc_);
       items_[t].sub_off = total_sub;
       items_[t].tiles_off = total_tiles;
+      items_[t].pt_off = total_points_;
       total_sub += pts_[t].nsub();
-      total_tiles += pts_[t].nsub() * tile_cells(groups_[items_[t].group].geom);
+      if (!bands_) total_tiles += pts_[t].nsub() * tile_cells(groups_[items_[t].group].geom);
       total_points_ += static_cast<std::size_t>(num_nu[t]);
     }
     tiles_.ensure(total_tiles ? total_tiles : 1, alloc_);
+    if constexpr (DIM == 2) {
+      if (bands_) {
+        total_bands_ = 0;
+        for (int t = 0; t < count_; ++t) {
+          const Group& g = groups_[items_[t].group];
+          rows_[t].build(pts_[t], ks_.ns, stream_, alloc_);
+          items_[t].band_off = total_bands_;
+          total_bands_ += static_cast<std::size_t>((g.n[1] + g.band.rows - 1) / g.band.rows);
+        }
+        sorted_.ensure(total_points_ ? total_points_ : 1, alloc_);
+        up_bjobs_.invalidate();
+      }
+    }
     for (int t = 0; t < count_; ++t) {
       const Item& it = items_[t];
       const Group& g = groups_[it.group];
@@ -228,6 +259,42 @@ public:
   // in / out: count device pointers each; modes contiguous (x fastest).
   void transform_type_1(const Cplx<T>* const* in, Cplx<T>* const* out) {
     if (!in || !out) throw neonufft::InputError("Neonufft:: null pointer table");
+    if constexpr (DIM == 2) {
+      if (bands_) {
+        for (int t = 0; t < count_; ++t) {
+          const Item& it = items_[t];
+          const Group& g = groups_[it.group];
+          BandJob<T> j{};
+          j.pts = rows_[t].recs();
+          j.perm = rows_[t].perm();
+          j.row_start = rows_[t].row_start();
+          j.in = in[t];
+          j.sorted = sorted_.data() + it.pt_off;
+          j.grid = grids_.data() + it.grid_off;
+          j.m = pts_[t].size();
+          j.pt_off = it.pt_off;
+          j.band_off = it.band_off;
+          j.n0 = static_cast<int>(g.n[0]);
+          j.n1 = static_cast<int>(g.n[1]);
+          j.rows = g.band.rows;
+          j.pitch = g.band.pitch;
+          if (fused_) {
+            j.h = g.pfft.h() + static_cast<std::size_t>(it.member) *
+                                   static_cast<std::size_t>(g.modes[0] * g.n[1]);
+            j.tw = tw_.data();
+            j.tw_len = static_cast<int>(tw_len_);
+            j.N0 = static_cast<int>(g.modes[0]);
+            j.log2n0 = log2_exact(g.n[0]);
+            j.log2rows = log2_exact(g.band.rows);
+          }
+          host_bjobs_[t] = j;
+        }
+        up_bjobs_.upload(bjobs_.data(), host_bjobs_.data(), stream_);
+        upload_io(reinterpret_cast<void* const*>(out));
+        run(1);
+        return;
+      }
+    }
     for (int t = 0; t < count_; ++t) {
       const Item& it = items_[t];
       host_sjobs_[t] = SpreadJob<T, DIM>{pts_[t].recs(), pts_[t].perm(), in[t], nullptr,
@@ -259,6 +326,8 @@ public:
   std::size_t grid_cells() const { return grids_.size(); }
   int count() const { return count_; }
   bool graph_active() const { return graph_ok_[0] || graph_ok_[1]; }
+  bool row_bands() const { return bands_; }
+  bool fused_fft() const { return fused_; }
   const KernelSetup& kernel() const { return ks_; }
 
 private:
@@ -268,7 +337,10 @@ private:
     std::size_t grid_off = 0;
     std::size_t sub_off = 0;
     std::size_t tiles_off = 0;  // first tile cell of this transform in tiles_
+    std::size_t pt_off = 0;     // first point of this transform over all transforms
+    std::size_t band_off = 0;   // first row band of this transform (row bands)
     int group = 0;
+    int member = 0;             // index of this transform inside its group
   };
   struct Group {
     IntType modes[3];
@@ -281,6 +353,7 @@ private:
     BinGeom geom{};
     CoverBuffers cover;
     std::size_t io_off = 0;  // first entry of this group's caller pointers in io_
+    BandLayout band;         // row-band layout (row-band spreading)
   };
 
   int order_fft() const { return opt_.order == NEONUFFT_MODE_ORDER_FFT ? 1 : 0; }
@@ -296,10 +369,19 @@ private:
 
   void enqueue(int type, hipStream_t s) {
     if (type == 1) {
-      launch_spread_batch<T, DIM>(ks_.ns, sjobs_.data(), subs_.data(), nsub_, max_lds_, ks_.es, s);
       // every fine-grid cell is written, so the grids need no zero fill
-      launch_tile_reduce_batch<T, DIM>(rjobs_.data(), count_, ks_.ns, regular_, max_bins_,
-                                       max_n0_, max_rows_, false, s);
+      if (bands_) {
+        if constexpr (DIM == 2) {
+          launch_band_gather<T>(bjobs_.data(), BandJob<T>{}, count_, total_points_, s);
+          launch_band_spread<T>(ks_.ns, bjobs_.data(), BandJob<T>{}, count_, total_bands_,
+                                band_lds_, false, fused_, ks_.es, s);
+        }
+      } else {
+        launch_spread_batch<T, DIM>(ks_.ns, sjobs_.data(), subs_.data(), nsub_, max_lds_, ks_.es,
+                                    s);
+        launch_tile_reduce_batch<T, DIM>(rjobs_.data(), count_, ks_.ns, regular_, max_bins_,
+                                         max_n0_, max_rows_, false, s);
+      }
       for (Group& g : groups_) {
         T* fs[3] = {g.fs[0].data(), g.fs[1].data(), g.fs[2].data()};
         IntType st[3] = {1, g.modes[0], g.modes[0] * g.modes[1]};
@@ -312,6 +394,12 @@ private:
                                           static_cast<int>(g.members.size()), s);
         } else {
           g.pfft.set_stream(s);
+          if constexpr (DIM == 2) {
+            if (fused_) {
+              g.pfft.forward_from_h(fs, order_fft(), nullptr, outs, st, s);
+              continue;
+            }
+          }
           g.pfft.forward(grids_.data() + g.offset, fs, order_fft(), nullptr, outs, st, s);
         }
       }
@@ -405,6 +493,16 @@ private:
   int max_n0_ = 0, max_rows_ = 0;
   std::size_t max_bins_ = 0;
   bool regular_ = true;  // every group's bins suit the bin-owned reduction
+  bool bands_ = true;    // 2D: row-band spreading for every group
+  bool fused_ = true;    // ... with the x-FFT inside the band kernel
+  IntType tw_len_ = 1;
+  DeviceBuffer<Cplx<T>> tw_;
+  std::size_t total_bands_ = 0, band_lds_ = 0;
+  std::vector<RowView<T>> rows_;  // row-sorted views (row bands)
+  DeviceBuffer<Cplx<T>> sorted_;  // strengths of all transforms in view order
+  DeviceBuffer<BandJob<T>> bjobs_;
+  std::vector<BandJob<T>> host_bjobs_;
+  TableUpload up_bjobs_;
   DeviceBuffer<Cplx<T>> tiles_;  // spreading tiles of all transforms
   DeviceBuffer<ReduceJob<T, DIM>> rjobs_;
   std::vector<ReduceJob<T, DIM>> host_rjobs_;

@@ -120,9 +120,9 @@ struct BatchHolder final : BatchIface {
   nnf::BatchPlanImpl<T, DIM> impl;
   BatchHolder(const neonufft::Options& o, int sign, IntType count, const int64_t* num_nu,
               const void* const* loc, const int64_t* modes, hipStream_t s,
-              std::shared_ptr<neonufft::Allocator> a, bool graph)
+              std::shared_ptr<neonufft::Allocator> a, bool graph, const nnf::PlanTuning& t)
       : impl(o, sign, count, num_nu, reinterpret_cast<const T* const*>(loc), modes, s,
-             std::move(a), graph) {}
+             std::move(a), graph, t) {}
   void set_points(const int64_t* n, const void* const* loc) override {
     impl.set_points(n, reinterpret_cast<const T* const*>(loc));
   }
@@ -144,8 +144,29 @@ struct BatchHolder final : BatchIface {
     i->num_shapes = impl.num_groups();
     i->grid_cells = static_cast<int64_t>(impl.grid_cells());
     i->graph = impl.graph_active() ? 1 : 0;
+    i->spread_mode = !impl.row_bands() ? NNF_SPREAD_TILES
+                     : impl.fused_fft() ? NNF_SPREAD_BANDS
+                                        : NNF_SPREAD_BANDS_GRID;
   }
 };
+
+nnf::PlanTuning to_tuning(const nnf_tuning* tuning) {
+  nnf::PlanTuning t;
+  if (tuning) {
+    for (int d = 0; d < 3; ++d) {
+      if (tuning->bin_size[d] < 0) throw neonufft::InputError("Neonufft:: negative bin size");
+      t.bin[d] = tuning->bin_size[d];
+    }
+    if (tuning->max_points < 0) throw neonufft::InputError("Neonufft:: negative max_points");
+    if (tuning->max_points > 0) t.max_points = static_cast<uint32_t>(tuning->max_points);
+    if (tuning->spread_mode < NNF_SPREAD_AUTO || tuning->spread_mode > NNF_SPREAD_BANDS_GRID)
+      throw neonufft::InputError("Neonufft:: unknown spread mode");
+    if (tuning->band_rows < 0) throw neonufft::InputError("Neonufft:: negative band rows");
+    t.spread_mode = tuning->spread_mode;
+    t.band_rows = tuning->band_rows;
+  }
+  return t;
+}
 
 neonufft::Options to_options(const nnf_options* o) {
   neonufft::Options opt;
@@ -264,15 +285,7 @@ NEONUFFT_EXPORT int nnf_plan_create_tuned(nnf_plan* plan, int precision, int dim
     if (!plan) throw neonufft::InputError("Neonufft:: null plan output pointer");
     *plan = nullptr;
     if (!modes) throw neonufft::InputError("Neonufft:: null modes");
-    nnf::PlanTuning t;
-    if (tuning) {
-      for (int d = 0; d < 3; ++d) {
-        if (tuning->bin_size[d] < 0) throw neonufft::InputError("Neonufft:: negative bin size");
-        t.bin[d] = tuning->bin_size[d];
-      }
-      if (tuning->max_points < 0) throw neonufft::InputError("Neonufft:: negative max_points");
-      if (tuning->max_points > 0) t.max_points = static_cast<uint32_t>(tuning->max_points);
-    }
+    const nnf::PlanTuning t = to_tuning(tuning);
     auto h = std::make_unique<nnf_plan_s>();
     h->p.reset(make_holder<PlanHolder, PlanIface>(
         precision, dim, to_options(opt), sign, static_cast<IntType>(num_nu), loc, modes,
@@ -492,20 +505,30 @@ NEONUFFT_EXPORT int nnf_autotune_clear_cache(void) {
   return guarded([&] { nnf::autotune_clear_cache(); });
 }
 
-NEONUFFT_EXPORT int nnf_batch_create(nnf_batch* batch, int precision, int dim,
-                                     const nnf_options* opt, int sign, int64_t count,
-                                     const int64_t* num_nu, const void* const* loc,
-                                     const int64_t* modes, void* stream, int use_graph) {
+NEONUFFT_EXPORT int nnf_batch_create_tuned(nnf_batch* batch, int precision, int dim,
+                                           const nnf_options* opt, int sign, int64_t count,
+                                           const int64_t* num_nu, const void* const* loc,
+                                           const int64_t* modes, void* stream, int use_graph,
+                                           const nnf_tuning* tuning) {
   return guarded([&] {
     if (!batch) throw neonufft::InputError("Neonufft:: null batch output pointer");
     *batch = nullptr;
     if (!modes || !num_nu) throw neonufft::InputError("Neonufft:: null modes or point counts");
+    const nnf::PlanTuning t = to_tuning(tuning);
     auto h = std::make_unique<nnf_batch_s>();
     h->p.reset(make_holder<BatchHolder, BatchIface>(
         precision, dim, to_options(opt), sign, static_cast<IntType>(count), num_nu, loc, modes,
-        static_cast<hipStream_t>(stream), default_alloc(), use_graph != 0));
+        static_cast<hipStream_t>(stream), default_alloc(), use_graph != 0, t));
     *batch = h.release();
   });
+}
+
+NEONUFFT_EXPORT int nnf_batch_create(nnf_batch* batch, int precision, int dim,
+                                     const nnf_options* opt, int sign, int64_t count,
+                                     const int64_t* num_nu, const void* const* loc,
+                                     const int64_t* modes, void* stream, int use_graph) {
+  return nnf_batch_create_tuned(batch, precision, dim, opt, sign, count, num_nu, loc, modes,
+                                stream, use_graph, nullptr);
 }
 
 NEONUFFT_EXPORT int nnf_batch_set_points(nnf_batch batch, const int64_t* num_nu,

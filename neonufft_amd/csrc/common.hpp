@@ -112,11 +112,13 @@ struct alignas(16) PointRec<float, 3> {
   float x[3];
   uint32_t cell;
 };
+// 2D records also keep the point's linear bin index (`bin`), from which the
+// row-sorted view of the row-band spreading is derived.
 template <>
 struct alignas(16) PointRec<float, 2> {
   float x[2];
   uint32_t cell;
-  uint32_t pad;
+  uint32_t bin;
 };
 template <>
 struct alignas(8) PointRec<float, 1> {
@@ -133,7 +135,8 @@ template <>
 struct alignas(16) PointRec<double, 2> {
   double x[2];
   uint32_t cell;
-  uint32_t pad[3];
+  uint32_t bin;
+  uint32_t pad[2];
 };
 template <>
 struct alignas(16) PointRec<double, 1> {

@@ -161,6 +161,55 @@ void launch_interp_batch(int ns, const InterpJob<T, DIM>* jobs, const Subprob* s
 template <typename T>
 std::size_t tile_bytes(const BinGeom& g);
 
+// ---- band.hip: row-band spreading (2D type 1) ------------------------------
+// Row-sorted view of a tile-sorted 2D point set: records sorted by their first
+// kernel row i0y (mod n1), with cell = (i0x + ns/2) | (row << 16) and
+// bin = row; perm_out maps the view back to the caller's indices.
+template <typename T>
+void launch_row_count(const PointRec<T, 2>* recs, std::size_t m, const BinGeom& g, int ns,
+                      uint32_t* count, uint32_t* rank, hipStream_t stream);
+template <typename T>
+void launch_row_scatter(const PointRec<T, 2>* recs, const uint32_t* perm, std::size_t m,
+                        const BinGeom& g, int ns, const uint32_t* row_start, const uint32_t* rank,
+                        PointRec<T, 2>* out, uint32_t* perm_out, hipStream_t stream);
+
+// One 2D transform of a row-band launch. Band b of the launch belongs to the
+// job with the largest band_off <= b and covers rows
+// [(b - band_off) * rows, + rows) of its n0 x n1 grid.
+template <typename T>
+struct BandJob {
+  const PointRec<T, 2>* pts;   // row-sorted records
+  const uint32_t* perm;        // row-sorted position -> caller's point index
+  const uint32_t* row_start;   // n1 + 1 offsets into the view
+  const Cplx<T>* in;           // caller's strengths
+  Cplx<T>* sorted;             // strengths in view order (gather output)
+  Cplx<T>* grid;               // fine grid, x fastest
+  std::size_t m;               // points
+  std::size_t pt_off;          // first point of this job in a flattened gather
+  std::size_t band_off;        // first band of this job in the launch
+  int n0, n1, rows, pitch;     // band rows; LDS row pitch (cells)
+  // fused x-FFT (n0 and rows powers of two, n0 >= 32): the retained x modes
+  // of every row go to h[x' * n1 + y] (not deconvolved)
+  Cplx<T>* h;
+  const Cplx<T>* tw;           // tw[N/2 + q] = exp(sign 2 pi i q / N), q < N/2, N <= tw_len
+  int tw_len, N0, log2n0, log2rows;
+};
+
+// Strengths into view order (sorted[p] = in[perm[p]]) for every job; `jobs`
+// is a device table of njobs entries sorted by pt_off, or null to use
+// `single`. total_points = sum of m.
+template <typename T>
+void launch_band_gather(const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
+                        std::size_t total_points, hipStream_t stream);
+// Forms (or adds to) every job's fine grid from its gathered strengths, one
+// workgroup per band; with `fft`, transforms each band along x in LDS and
+// writes the retained modes to job.h instead (no grid, no accumulate).
+template <typename T>
+void launch_band_spread(int ns, const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
+                        std::size_t total_bands, std::size_t lds_bytes, bool accumulate, bool fft,
+                        const EsParams& es, hipStream_t stream);
+std::size_t band_lds_bytes(int rows, int pitch, std::size_t acc_bytes);
+
 // ---- grid.hip: deconvolution factors, up/down-sampling, phases -----------
 template <typename T>
 void launch_fill_zero(Cplx<T>* p, std::size_t count, hipStream_t stream);

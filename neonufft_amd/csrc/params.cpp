@@ -129,6 +129,16 @@ KernelSetup make_kernel_setup(double tol, double upsampfac, bool use_horner) {
   return ks;
 }
 
+int band_rows(IntType n0, IntType pitch, IntType n1, int ns, std::size_t acc_bytes,
+              std::size_t budget, int min_bands) {
+  if (n0 + ns / 2 > 65535 || n1 > 65535 || n1 < ns) return 0;
+  const std::size_t row_bytes = 2 * static_cast<std::size_t>(pitch) * acc_bytes;
+  IntType rows = static_cast<IntType>(budget / row_bytes);
+  rows = std::min<IntType>(rows, n1 - ns + 1);
+  if (min_bands > 1) rows = std::min<IntType>(rows, std::max<IntType>(2, n1 / min_bands));
+  return rows >= 1 ? static_cast<int>(rows) : 0;
+}
+
 BinGeom choose_bins(int dim, const IntType* n, int ns, std::size_t elem_bytes,
                     const int* bin_override) {
   BinGeom g{};

@@ -43,6 +43,19 @@ double fit_horner(int ns, double beta, int deg, double* coef /* ns*(deg+1) */);
 BinGeom choose_bins(int dim, const IntType* n, int ns, std::size_t elem_bytes,
                     const int* bin_override = nullptr);
 
+// Rows per band of the row-band spreading of an n0 x n1 grid with LDS rows of
+// `pitch` cells: as many as two accumulator planes (acc_bytes per cell) of
+// `budget` LDS bytes hold, at most n1 - ns + 1 (the point range of a band must
+// not wrap onto itself) and at most max(2, n1 / min_bands) so that a grid
+// yields about min_bands bands; 0 when the grid does not suit the row view
+// (n0 + ns/2 or n1 above 65535, or one row over budget).
+int band_rows(IntType n0, IntType pitch, IntType n1, int ns, std::size_t acc_bytes,
+              std::size_t budget, int min_bands);
+
+// LDS bytes of one row band (8 rows of a 2048-cell fp32 grid with the fused
+// FFT's padded pitch fit).
+constexpr std::size_t kBandBudget = 136 * 1024;
+
 // Default number of points per subproblem.
 constexpr uint32_t kDefaultMaxPoints = 2048;
 

@@ -24,10 +24,79 @@
 
 namespace nnf {
 
+// Spreading decomposition of 2D type 1 (NNF_SPREAD_* of the C ABI).
+enum SpreadMode { kSpreadAuto = 0, kSpreadTiles = 1, kSpreadBands = 2, kSpreadBandsGrid = 3 };
+
 struct PlanTuning {
   int bin[3] = {0, 0, 0};  // 0 = default
   uint32_t max_points = kDefaultMaxPoints;
+  int spread_mode = kSpreadAuto;
+  int band_rows = 0;  // 0 = default
 };
+
+// Row-band layout of a 2D plan: rows = 0 when subgrid tiles are used (asked
+// for, or the grid does not suit the row view). fft: type 1 runs the x-FFT
+// inside the band kernel (n0 a power of two >= 32, rows a power of two; LDS
+// rows padded by 8 words so that the transposed mode write-out spreads over
+// the banks).
+struct BandLayout {
+  int rows = 0;
+  int pitch = 0;
+  bool fft = false;
+};
+
+inline BandLayout plan_band_layout(const IntType* n, int ns, std::size_t acc_bytes,
+                                   const PlanTuning& t, int min_bands) {
+  BandLayout b;
+  if (t.spread_mode == kSpreadTiles) return b;
+  const IntType n0 = n[0];
+  const bool pow2 = n0 >= 32 && (n0 & (n0 - 1)) == 0;
+  b.fft = pow2 && t.spread_mode != kSpreadBandsGrid;
+  b.pitch = static_cast<int>(b.fft ? n0 + 8 : n0);
+  int rows = band_rows(n0, b.pitch, n[1], ns, acc_bytes, kBandBudget, min_bands);
+  if (rows > 0 && t.band_rows > 0) {
+    rows = std::min(band_rows(n0, b.pitch, n[1], ns, acc_bytes, kBandBudget, 1), t.band_rows);
+  }
+  if (b.fft && rows > 0) {
+    int p = 1;
+    while (2 * p <= rows) p *= 2;
+    rows = p;
+  }
+  if (rows <= 0) {
+    if (t.spread_mode == kSpreadBands || t.spread_mode == kSpreadBandsGrid)
+      throw neonufft::InputError("Neonufft:: grid does not suit row-band spreading");
+    return BandLayout{};
+  }
+  b.rows = rows;
+  return b;
+}
+
+inline int log2_exact(IntType v) {
+  int l = 0;
+  while ((IntType(1) << l) < v) ++l;
+  return l;
+}
+
+// Twiddles of every power-of-two length N <= len, level by level:
+// tw[N/2 + q] = exp(sign 2 pi i q / N) for q < N/2 (tw[0] unused), in FP64.
+template <typename T>
+void upload_twiddles(DeviceBuffer<Cplx<T>>& tw, IntType len, int sign, hipStream_t stream,
+                     const std::shared_ptr<neonufft::Allocator>& alloc) {
+  std::vector<Cplx<T>> h(static_cast<std::size_t>(std::max<IntType>(len, 2)));
+  h[0] = Cplx<T>{T(1), T(0)};
+  for (IntType N = 2; N <= len; N *= 2) {
+    for (IntType q = 0; q < N / 2; ++q) {
+      const double a = 2.0 * kPi * static_cast<double>(q) / static_cast<double>(N);
+      h[static_cast<std::size_t>(N / 2 + q)] =
+          Cplx<T>{static_cast<T>(std::cos(a)), static_cast<T>(sign * std::sin(a))};
+    }
+  }
+  tw.reset(h.size(), alloc);
+  check_hip(hipMemcpyAsync(tw.data(), h.data(), h.size() * sizeof(Cplx<T>), hipMemcpyHostToDevice,
+                           stream),
+            "hipMemcpyAsync(twiddles)");
+  check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
 
 template <typename T, int DIM>
 class PlanImpl {
@@ -63,6 +132,11 @@ public:
     }
     geom_ = choose_bins(DIM, n_, ks_.ns, sizeof(Cplx<T>), tune_.bin);
     cover_.build(DIM, geom_, ks_.ns, stream_, alloc_);
+    if constexpr (DIM == 2) {
+      band_ = plan_band_layout(n_, ks_.ns, sizeof(T), tune_, 256);
+      band_rows_ = band_.rows;
+      if (band_.fft) upload_twiddles<T>(tw_, n_[0], sign_, stream_, alloc_);
+    }
     set_points(num_nu, loc);
   }
 
@@ -73,12 +147,23 @@ public:
     FoldParams fp{{0, 0, 0}, {1, 1, 1}};
     pts_.build(l, static_cast<std::size_t>(num_nu), fp, geom_, ks_.ns, tune_.max_points, stream_,
                alloc_);
-    tiles_.ensure(std::max<std::size_t>(pts_.nsub(), 1) * tile_cells(geom_), alloc_);
+    if constexpr (DIM == 2) {
+      if (band_rows_ > 0) {
+        rows_.build(pts_, ks_.ns, stream_, alloc_);
+        sorted_.ensure(std::max<std::size_t>(pts_.size(), 1), alloc_);
+      } else {
+        tiles_.ensure(std::max<std::size_t>(pts_.nsub(), 1) * tile_cells(geom_), alloc_);
+      }
+    } else {
+      tiles_.ensure(std::max<std::size_t>(pts_.nsub(), 1) * tile_cells(geom_), alloc_);
+    }
     check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   }
 
-  // Stage 1 of type 1: spread into per-subproblem tiles, then form the fine
-  // grid from them (overwriting it, or adding to it when accumulating).
+  // Stage 1 of type 1: form the fine grid from the strengths (overwriting
+  // it, or adding to it when accumulating). Subgrid tiles: spread into
+  // per-subproblem tiles, then reduce them into the grid. Row bands (2D):
+  // gather the strengths into row order, then spread band by band.
   void spread(const Cplx<T>* in, bool accumulate = false) {
     spread_tiles(in);
     reduce_tiles(accumulate);
@@ -87,10 +172,25 @@ public:
   // The two kernels of spread(), separately (for stage timing).
   void spread_tiles(const Cplx<T>* in) {
     if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
+    if constexpr (DIM == 2) {
+      if (band_rows_ > 0) {
+        launch_band_gather<T>(nullptr, band_job(in), 1, pts_.size(), stream_);
+        return;
+      }
+    }
     launch_spread<T, DIM>(ks_.ns, pts_.recs(), pts_.perm(), pts_.subs(), pts_.nsub(), in, nullptr,
                           tiles_.data(), geom_, ks_.es, stream_);
   }
   void reduce_tiles(bool accumulate) {
+    if constexpr (DIM == 2) {
+      if (band_rows_ > 0) {
+        const int nb = static_cast<int>((n_[1] + band_rows_ - 1) / band_rows_);
+        launch_band_spread<T>(ks_.ns, nullptr, band_job(nullptr), 1, static_cast<std::size_t>(nb),
+                              band_lds_bytes(band_rows_, band_.pitch, sizeof(T)), accumulate,
+                              false, ks_.es, stream_);
+        return;
+      }
+    }
     const ReduceJob<T, DIM> job{tiles_.data(), pts_.sub_starts(), grid_.data(), geom_,
                                 cover_.table()};
     launch_tile_reduce<T, DIM>(job, ks_.ns, accumulate, stream_);
@@ -105,8 +205,22 @@ public:
 
   void transform_type_1(const Cplx<T>* in, Cplx<T>* out, const IntType* out_strides) {
     if (!out || !out_strides) throw neonufft::InputError("Neonufft:: null output");
-    spread(in);
     T* fs[3] = {fs_[0].data(), fs_[1].data(), fs_[2].data()};
+    if constexpr (DIM == 2) {
+      if (band_.fft) {
+        // gather, then spread + x-FFT per band straight into the pruned
+        // FFT's intermediate; y-FFT and deconvolution follow
+        if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
+        launch_band_gather<T>(nullptr, band_job(in), 1, pts_.size(), stream_);
+        const int nb = static_cast<int>((n_[1] + band_rows_ - 1) / band_rows_);
+        launch_band_spread<T>(ks_.ns, nullptr, band_job(nullptr), 1, static_cast<std::size_t>(nb),
+                              band_lds_bytes(band_rows_, band_.pitch, sizeof(T)), false, true,
+                              ks_.es, stream_);
+        pfft_.forward_from_h(fs, order_fft(), out, nullptr, out_strides, stream_);
+        return;
+      }
+    }
+    spread(in);
     if constexpr (DIM == 1) {
       fft_.execute(grid_.data());
       launch_downsample<T, DIM>(order_fft(), grid_.data(), n_, fs, out, out_strides, modes_,
@@ -145,6 +259,8 @@ public:
     info->num_points = static_cast<int64_t>(pts_.size());
     info->horner_degree = ks_.horner_deg;
     info->quad_nodes = ks_.quad.nq;
+    info->spread_mode = band_rows_ == 0 ? kSpreadTiles : (band_.fft ? kSpreadBands : kSpreadBandsGrid);
+    info->band_rows = band_rows_;
   }
 
   Cplx<T>* fine_grid() const { return grid_.data(); }
@@ -152,6 +268,30 @@ public:
 
 private:
   int order_fft() const { return opt_.order == NEONUFFT_MODE_ORDER_FFT ? 1 : 0; }
+
+  BandJob<T> band_job(const Cplx<T>* in) const {
+    BandJob<T> j{};
+    j.pts = rows_.recs();
+    j.perm = rows_.perm();
+    j.row_start = rows_.row_start();
+    j.in = in;
+    j.sorted = sorted_.data();
+    j.grid = grid_.data();
+    j.m = pts_.size();
+    j.n0 = static_cast<int>(n_[0]);
+    j.n1 = static_cast<int>(n_[1]);
+    j.rows = band_rows_;
+    j.pitch = band_.pitch;
+    if (band_.fft) {
+      j.h = pfft_.h();
+      j.tw = tw_.data();
+      j.tw_len = static_cast<int>(n_[0]);
+      j.N0 = static_cast<int>(modes_[0]);
+      j.log2n0 = log2_exact(n_[0]);
+      j.log2rows = log2_exact(band_rows_);
+    }
+    return j;
+  }
 
   neonufft::Options opt_;
   int sign_;
@@ -162,6 +302,11 @@ private:
   IntType modes_[3];
   IntType n_[3];
   BinGeom geom_{};
+  int band_rows_ = 0;             // 2D row-band spreading (0 = subgrid tiles)
+  BandLayout band_;
+  DeviceBuffer<Cplx<T>> tw_;      // x-FFT twiddles of the fused type 1
+  RowView<T> rows_;               // row-sorted view (row bands)
+  DeviceBuffer<Cplx<T>> sorted_;  // strengths in view order (row bands)
   DeviceBuffer<Cplx<T>> grid_;
   DeviceBuffer<Cplx<T>> tiles_;  // spreading tiles, tile_cells(geom_) per subproblem
   CoverBuffers cover_;

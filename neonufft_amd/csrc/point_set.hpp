@@ -83,4 +83,62 @@ private:
   DeviceBuffer<Subprob> subs_;
 };
 
+// Row-sorted view of a 2D point set for the row-band spreading: the records
+// sorted by their first kernel row (counting sort over the n1 rows, built
+// from the tile-sorted records on the device, no host synchronisation), then
+// bank-ordered inside chunks of at most kMaxSubproblemPoints points of a row
+// (consecutive points start on distinct cells mod 32, so the lanes of a wave
+// hit distinct LDS banks).
+template <typename T>
+class RowView {
+public:
+  void build(const PointSet<T, 2>& ps, int ns, hipStream_t stream,
+             const std::shared_ptr<neonufft::Allocator>& alloc) {
+    const BinGeom& g = ps.geom();
+    const std::size_t n1 = static_cast<std::size_t>(g.n[1]);
+    const std::size_t m = ps.size();
+    m_ = m;
+    count_.ensure(n1 + 1, alloc);
+    start_.ensure(n1 + 1, alloc);
+    nchunk_.ensure(n1 + 1, alloc);
+    chunk_start_.ensure(n1 + 1, alloc);
+    scratch_.ensure(scan_scratch_size(n1 + 1), alloc);
+    rank_.ensure(m ? m : 1, alloc);
+    recs_.ensure(m ? m : 1, alloc);
+    perm_.ensure(m ? m : 1, alloc);
+    // upper bound on the chunks: one partial chunk per row plus the full ones
+    const std::size_t max_chunks = n1 + m / kMaxSubproblemPoints + 1;
+    chunks_.ensure(max_chunks, alloc);
+    check_hip(hipMemsetAsync(count_.data(), 0, (n1 + 1) * sizeof(uint32_t), stream),
+              "hipMemsetAsync(row counts)");
+    check_hip(hipMemsetAsync(chunks_.data(), 0, max_chunks * sizeof(Subprob), stream),
+              "hipMemsetAsync(row chunks)");
+    launch_row_count<T>(ps.recs(), m, g, ns, count_.data(), rank_.data(), stream);
+    launch_exclusive_scan(count_.data(), start_.data(), n1 + 1, scratch_.data(), stream);
+    launch_row_scatter<T>(ps.recs(), ps.perm(), m, g, ns, start_.data(), rank_.data(),
+                          recs_.data(), perm_.data(), stream);
+    launch_subprob_counts(count_.data(), n1 + 1, kMaxSubproblemPoints, nchunk_.data(), stream);
+    launch_exclusive_scan(nchunk_.data(), chunk_start_.data(), n1 + 1, scratch_.data(), stream);
+    launch_subprob_build(count_.data(), start_.data(), chunk_start_.data(), n1,
+                         kMaxSubproblemPoints, chunks_.data(), stream);
+    // residue of a view record = (i0x + ns/2) mod 32 (tile pitch 0 drops the row)
+    BinGeom og{};
+    og.tile[0] = 0;
+    og.tile[1] = 0;
+    og.tile[2] = 1;
+    launch_bank_order<T, 2>(chunks_.data(), max_chunks, recs_.data(), perm_.data(), og, stream);
+  }
+
+  const PointRec<T, 2>* recs() const { return recs_.data(); }
+  const uint32_t* perm() const { return perm_.data(); }
+  const uint32_t* row_start() const { return start_.data(); }
+  std::size_t size() const { return m_; }
+
+private:
+  std::size_t m_ = 0;
+  DeviceBuffer<PointRec<T, 2>> recs_;
+  DeviceBuffer<uint32_t> perm_, rank_, count_, start_, nchunk_, chunk_start_, scratch_;
+  DeviceBuffer<Subprob> chunks_;
+};
+
 }  // namespace nnf

@@ -88,6 +88,7 @@ __global__ void __launch_bounds__(kBlock)
               << (CellPack<DIM>::bits == 32 ? 0 : d * CellPack<DIM>::bits);
     }
     rec.cell = cell;
+    if constexpr (DIM == 2) rec.bin = bin;
     const uint32_t pos = bin_start[bin] + rank[i];
     pts[pos] = rec;
     perm[pos] = static_cast<uint32_t>(i);

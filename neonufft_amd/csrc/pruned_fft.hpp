@@ -70,6 +70,22 @@ public:
     }
   }
 
+  // Intermediate of the 2D forward transform: per grid, N0 x-modes (retained
+  // order) of n1 cells each, y fastest. A caller that produces it directly
+  // (row-band spreading with its own x-FFT) continues with forward_from_h.
+  Cplx<T>* h() const { return h_.data(); }
+
+  // 2D type 1 from a filled h(): y-FFT and deconvolution into the outputs.
+  void forward_from_h(T* const* fs, int order_fft, Cplx<T>* single, Cplx<T>* const* outs,
+                      const IntType* st, hipStream_t s) {
+    static_assert(DIM == 2, "2D only");
+    const long long n1 = n_[1], N0 = N_[0], N1 = N_[1];
+    py_.execute(h_.data());
+    launch_modes_out<T, 2>(h_.data(), N0 * n1, single, outs, count_, static_cast<int>(N0), 1,
+                           static_cast<int>(n1), static_cast<int>(N1), st[0], 0, st[1], fs[0],
+                           fs[1], fs[1], order_fft, s);
+  }
+
   void set_stream(hipStream_t s) {
     px_.set_stream(s);
     py_.set_stream(s);

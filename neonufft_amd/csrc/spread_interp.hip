@@ -15,7 +15,7 @@
 //  - kernel weights come from per-piece Horner polynomials in registers (fp32)
 //    or the exact exp/sqrt formula (fp64);
 //  - subproblem -> workgroup mapping keeps neighbouring bins on one XCD.
-#include "kernels.hpp"
+#include "es_device.hpp"
 
 namespace nnf {
 
@@ -33,55 +33,6 @@ constexpr int kInterpThreads = sizeof(T) == 4 ? 1024 : 512;
 // points per thread in one spreading subproblem
 constexpr int kPointsPerThread = kMaxSubproblemPoints / kThreads;
 static_assert(kPointsPerThread * kThreads == kMaxSubproblemPoints);
-
-// Map the hardware workgroup id to a subproblem so that consecutive
-// subproblems (neighbouring bins) run on the same XCD: hardware dispatches
-// workgroup b to XCD b % 8.
-__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t n) {
-  const uint32_t n8 = n & ~7u;
-  if (b >= n8) return b;
-  const uint32_t per = n8 >> 3;
-  return (b & 7u) * per + (b >> 3);
-}
-
-__device__ inline int wrap_once(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
-
-// Weight of kernel piece j (cell i0 + j) for Horner argument x in [0,1):
-// phi(x + j - NS/2).
-template <int NS>
-__device__ inline float es_piece(float x, int j, const EsParams& es) {
-  constexpr int DEG = horner_degree(NS);
-  const float y = fmaf(2.0f, x, -1.0f);
-  const float* c = es.coef + j * (DEG + 1);
-  float v = c[DEG];
-#pragma unroll
-  for (int d = DEG - 1; d >= 0; --d) v = fmaf(v, y, c[d]);
-  return v;
-}
-
-template <int NS>
-__device__ inline double es_piece(double x, int j, const EsParams& es) {
-  constexpr double c = 4.0 / (NS * NS);
-  const double t = x + static_cast<double>(j) - 0.5 * NS;
-  const double a = 1.0 - c * t * t;
-  return a > 0.0 ? exp(es.beta * (sqrt(a) - 1.0)) : 0.0;
-}
-
-template <typename T, int NS>
-__device__ inline void es_weights(T x, const EsParams& es, T (&k)[NS]) {
-#pragma unroll
-  for (int j = 0; j < NS; ++j) k[j] = es_piece<NS>(x, j, es);
-}
-
-template <int DIM>
-__device__ inline int unpack_cell(uint32_t cell, int d) {
-  if constexpr (CellPack<DIM>::bits == 32) {
-    return static_cast<int>(cell);
-  } else {
-    return static_cast<int>((cell >> (d * CellPack<DIM>::bits)) & CellPack<DIM>::mask);
-  }
-}
-
 
 // Tile geometry of one subproblem.
 template <int DIM, int NS>
@@ -108,45 +59,6 @@ struct TileInfo {
     t1 = DIM >= 2 ? g.tile[1] : 1;
   }
 };
-
-// ---------------------------------------------------------------------------
-// Spreading
-//
-// The tile accumulates in fixed point with integer LDS atomics: on gfx950,
-// ds_add_u32 sustains 5-9e12 lane-ops/s while ds_add_f32 runs at ~2e11.
-// Each subproblem picks its own scale from a rigorous bound on any tile value,
-// bound = sum over its points of |Re c| + |Im c| (kernel weights are <= 1), so
-// fp32 plans keep 30 fractional-range bits in int32 and fp64 plans 62 bits in
-// int64. The flush converts back and adds to the grid with float atomics.
-// ---------------------------------------------------------------------------
-template <typename T>
-struct FixedAcc;
-template <>
-struct FixedAcc<float> {
-  using type = int;
-  static constexpr int bits = 30;
-};
-template <>
-struct FixedAcc<double> {
-  using type = long long;
-  static constexpr int bits = 62;
-};
-
-// Round to the nearest integer (floor(v + 0.5)) in one VALU instruction; the
-// compiler emits rndne + cvt (two) for __float2int_rn. |v| <= 2^30 here.
-__device__ inline int to_fixed(float v) {
-  int r;
-  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
-  return r;
-}
-__device__ inline long long to_fixed(double v) { return __double2ll_rn(v); }
-
-__device__ inline void lds_add_fixed(int* p, int v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ inline void lds_add_fixed(long long* p, long long v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
 
 // Sum over the workgroup; contains a __syncthreads().
 template <typename T>

@@ -248,3 +248,58 @@ def test_type3_mixed(case):
     assert alg_err < 20 * tol
     assert err < 1.2 * alg_err + 2e-5, (err, alg_err)
     assert O.rel_l2(got, alg) < 5e-5 + 0.05 * tol
+
+
+# 2D type 1 through each spreading decomposition: row bands with the x-FFT
+# fused into the band kernel (power-of-two n0 >= 32), row bands writing the
+# fine grid, and subgrid tiles. All three must give the direct sum's answer
+# to the plan's tolerance and agree with each other to rounding.
+BAND_T1 = [
+    # (modes, M, tol, sign, band_rows)
+    ((64, 48), 3000, 1e-5, -1, 0),
+    ((16, 100), 2000, 1e-5, 1, 0),
+    ((512, 40), 5000, 1e-5, -1, 0),
+    ((1024, 12), 4000, 1e-4, 1, 0),
+    ((64, 48), 3000, 1e-5, -1, 1),
+    ((64, 48), 3000, 1e-6, 1, 2),
+    ((128, 30), 1000, 1e-9, -1, 0),
+]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("order", [nnf.ModeOrder.CMCL, nnf.ModeOrder.FFT])
+@pytest.mark.parametrize("case", BAND_T1,
+                         ids=[f"m{'x'.join(map(str, c[0]))}_n{c[1]}_tol{c[2]:g}_s{c[3]}_r{c[4]}"
+                              for c in BAND_T1])
+def test_type1_spread_modes(case, order, prec):
+    modes, m, tol, sign, rows = case
+    if prec == "f32" and tol < 1e-7:
+        pytest.skip("below fp32 resolution")
+    rng = np.random.default_rng(hash(case) & 0xFFFF)
+    x = points(rng, 2, m, prec)
+    c = cplx(rng, m, prec)
+    opt = nnf.Options(tol=tol, upsampfac=2.0, order=order)
+    loc = [dev(v, RDT[prec]) for v in x]
+    cin = dev(c, CDT[prec])
+    outs = {}
+    for mode in ("bands", "bands_grid", "tiles"):
+        plan = nnf.Plan(opt, sign, m, loc, modes, spread_mode=mode,
+                        band_rows=rows if mode != "tiles" else 0)
+        assert plan.info()["spread_mode"] == mode
+        out = torch.zeros(int(np.prod(modes)), dtype=CDT[prec], device="cuda")
+        plan.transform_type_1(cin, out)
+        plan.transform_type_1(cin, out)  # a second call gives the same result
+        torch.cuda.synchronize()
+        outs[mode] = out.cpu().numpy().astype(np.complex128)
+    xs = [v.astype(np.float64) for v in x]
+    idx = sample_modes(modes, rng)
+    offt = order == nnf.ModeOrder.FFT
+    ref = O.direct_type1(xs, c.astype(np.complex128), O.mode_vectors(modes, idx, offt), sign)
+    amp = amplification(modes, tol, 2.0)
+    alg = O.nufft1(xs, c.astype(np.complex128), modes, sign, tol, 2.0, order_fft=offt)
+    alg_err = O.rel_l2(alg[idx], ref)
+    for mode, got in outs.items():
+        assert O.rel_l2(got[idx], ref) < bound(prec, alg_err, 1, amp), mode
+    same = 1e-12 if prec == "f64" else 5e-6
+    assert O.rel_l2(outs["bands"], outs["tiles"]) < same
+    assert O.rel_l2(outs["bands_grid"], outs["tiles"]) < same

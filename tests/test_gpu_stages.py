@@ -7,7 +7,10 @@ interpolation_kernel.cu:27-319).
 
 Geometries cover both forms of the tile reduction: regular bins (every cell
 covered by its own bin's tile and at most one neighbour per dimension) and
-small grids where one tile wraps onto itself (general cover tables).
+small grids where one tile wraps onto itself (general cover tables). 2D plans
+are also run with row-band spreading (one workgroup per band of whole grid
+rows) for several band heights, down to one row and up to the largest band a
+grid allows (n1 - ns + 1 rows, whose point range wraps below row 0).
 """
 import ctypes
 
@@ -81,11 +84,12 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def _plan(prec, dim, modes, m, seed):
+def _plan(prec, dim, modes, m, seed, **tuning):
     rng = np.random.default_rng(seed)
     x = points(rng, dim, m, prec)
     opt = nnf.Options(tol=1e-5 if prec == "f32" else 1e-10)
-    plan = nnf.Plan(opt, -1, m, [dev(v, RDT[prec]) for v in x], modes, dtype=RDT[prec])
+    plan = nnf.Plan(opt, -1, m, [dev(v, RDT[prec]) for v in x], modes, dtype=RDT[prec],
+                    **tuning)
     info = plan.info()
     u = [O.fold(v.astype(np.float64)) for v in x]
     return rng, plan, info, u
@@ -122,6 +126,66 @@ def test_spread_and_accumulate(case, prec):
     plan.spread(dev(c2, CDT[prec]))
     got2b = d2h(plan.fine_grid_ptr(), cells, got1.dtype)
     assert np.array_equal(got2, got2b)
+
+
+# 2D grids for the row-band spreading: (modes, points, band rows; 0 = default,
+# -1 = the largest band the grid allows)
+BAND_CASES = [
+    ((128, 100), 5000, 0),
+    ((128, 100), 5000, 1),
+    ((128, 100), 5000, 3),
+    ((128, 100), 5000, -1),
+    ((33, 47), 2000, 5),
+    ((23, 2), 100, 0),
+    ((23, 2), 100, -1),
+    ((256, 64), 0, 0),
+    ((4, 300), 3000, 7),
+]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("case", BAND_CASES,
+                         ids=[f"{'x'.join(map(str, c[0]))}_m{c[1]}_r{c[2]}" for c in BAND_CASES])
+def test_band_spread(case, prec):
+    modes, m, rows = case
+    ns = 6 if prec == "f32" else 11
+    n0 = max(2 * ns, 2 * modes[0])
+    n1 = max(2 * ns, 2 * modes[1])
+    fused = n0 >= 32 and (n0 & (n0 - 1)) == 0
+    if rows == -1:
+        rows = n1 - ns + 1
+    # requested rows, capped by the 136 KiB LDS band (two int32 / int64
+    # accumulator planes per row of n0 words, n0 + 8 with the fused x-FFT)
+    # and rounded down to a power of two with the fused x-FFT
+    pitch = n0 + 8 if fused else n0
+    expect = min(rows, n1 - ns + 1, (136 * 1024) // (2 * pitch * (4 if prec == "f32" else 8)))
+    if fused and expect > 0:
+        expect = 1 << (expect.bit_length() - 1)
+    rng, plan, info, u = _plan(prec, 2, modes, m, 5, spread_mode="bands", band_rows=rows)
+    assert info["spread_mode"] == ("bands" if fused else "bands_grid")
+    if rows:
+        assert info["band_rows"] == expect
+    n, ns, beta = info["grid"], info["n_spread"], info["es_beta"]
+    cells = int(np.prod(n))
+    c1 = cplx(rng, m, prec)
+    c2 = cplx(rng, m, prec)
+    plan.spread(dev(c1, CDT[prec]))
+    got1 = d2h(plan.fine_grid_ptr(), cells, NPC[prec])
+    if m == 0:
+        assert not np.any(got1)
+        return
+    ref1 = O.spread(n, ns, beta, u, c1.astype(np.complex128)).ravel()
+    assert _rel(got1, ref1) < TOL[prec]
+    plan.spread(dev(c2, CDT[prec]), accumulate=True)
+    got12 = d2h(plan.fine_grid_ptr(), cells, NPC[prec])
+    ref12 = O.spread(n, ns, beta, u, (c1 + c2).astype(np.complex128)).ravel()
+    assert _rel(got12, ref12) < TOL[prec]
+    # the same plan with subgrid tiles spreads the same grid
+    _, tplan, tinfo, _ = _plan(prec, 2, modes, m, 5, spread_mode="tiles")
+    assert tinfo["spread_mode"] == "tiles"
+    tplan.spread(dev(c1, CDT[prec]))
+    got_t = d2h(tplan.fine_grid_ptr(), cells, NPC[prec])
+    assert _rel(got1, got_t) < TOL[prec]
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
