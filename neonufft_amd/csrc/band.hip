@@ -137,7 +137,7 @@ __device__ inline T wave_max(T v) {
 //            lanes take consecutive positions, conflict-free), plus one
 //            radix-2 pass at half-span 32 when their count is odd; a thread
 //            keeps one position across the rows of the band, so it loads its
-//            twiddles once per pass and overlaps the loads of two rows;
+//            twiddles once per pass;
 //   stage A: half-spans 16 .. 1 inside aligned 32-cell blocks: a quad of
 //            lanes owns a block, 8 consecutive positions per lane (16 and 8
 //            by DPP quad permutations, 4, 2, 1 in registers).
@@ -149,6 +149,25 @@ __device__ inline T wave_max(T v) {
 template <typename T>
 __device__ inline Cplx<T> twid(const Cplx<T>* __restrict__ tw, int n, int q) {
   return tw[(n >> 1) + q];
+}
+
+// cos and sin of 2 pi j / 32, j < 16: the twiddles of the in-register
+// levels (W_32^j, W_16^j = W_32^2j, W_8^j = W_32^4j, W_4^j = W_32^8j) as
+// literals, so they occupy no registers.
+constexpr double kCos32[16] = {
+    1.0, 0.9807852804032304, 0.9238795325112867, 0.8314696123025452,
+    0.7071067811865476, 0.5555702330196023, 0.38268343236508984, 0.19509032201612833,
+    6.123233995736766e-17, -0.1950903220161282, -0.3826834323650897, -0.555570233019602,
+    -0.7071067811865475, -0.8314696123025453, -0.9238795325112867, -0.9807852804032304};
+constexpr double kSin32[16] = {
+    0.0, 0.19509032201612825, 0.3826834323650898, 0.5555702330196022,
+    0.7071067811865475, 0.8314696123025452, 0.9238795325112867, 0.9807852804032304,
+    1.0, 0.9807852804032304, 0.9238795325112867, 0.8314696123025455,
+    0.7071067811865476, 0.5555702330196022, 0.3826834323650899, 0.1950903220161286};
+
+template <typename T>
+__device__ inline Cplx<T> w32(int j, T sgn) {
+  return Cplx<T>{static_cast<T>(kCos32[j]), sgn * static_cast<T>(kSin32[j])};
 }
 
 // (a, b) <- (a + b, (a - b) w)
@@ -238,12 +257,10 @@ __device__ inline void dif4_pass(const Acc* ire, const Acc* iim, T* fre, T* fim,
     const int p = (j - q) * 4 + q;
     const Cplx<T> w40 = twid(tw, 4 * u, q), w41 = twid(tw, 4 * u, q + u);
     const Cplx<T> w2 = twid(tw, 2 * u, q);
-    int r = r0;
-    for (; r + rstep < rk; r += 2 * rstep) {
+    // (unrolling two rows here spills registers at 1024 threads)
+    for (int r = r0; r < rk; r += rstep) {
       dif4<CONV>(ire, iim, fre, fim, r * pitch + p, u, w40, w41, w2, inv);
-      dif4<CONV>(ire, iim, fre, fim, (r + rstep) * pitch + p, u, w40, w41, w2, inv);
     }
-    if (r < rk) dif4<CONV>(ire, iim, fre, fim, r * pitch + p, u, w40, w41, w2, inv);
   }
 }
 
@@ -275,7 +292,7 @@ __device__ inline void dif2_pass32(const Acc* ire, const Acc* iim, T* fre, T* fi
 
 template <bool CONV, typename T, int NT, typename Acc>
 __device__ inline void dif_stage_a(const Acc* ire, const Acc* iim, T* fre, T* fim, int rk,
-                                   int pitch, int log2n0, T inv, const Cplx<T>* __restrict__ tw) {
+                                   int pitch, int log2n0, T inv, T sgn) {
   const int lc = log2n0 - 3;  // log2 chunks of 8 per row
   const int chunks = 1 << lc;
   const int rstep = chunks >= NT ? 1 : NT >> lc;
@@ -294,19 +311,20 @@ __device__ inline void dif_stage_a(const Acc* ire, const Acc* iim, T* fre, T* fi
       }
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        dif_quad<2>(xr[e], xi[e], (c & 2) == 0, twid(tw, 32, 8 * (c & 1) + e));
+        const Cplx<T> lo = w32<T>(e, sgn), hi = w32<T>(8 + e, sgn);
+        dif_quad<2>(xr[e], xi[e], (c & 2) == 0, (c & 1) ? hi : lo);
       }
 #pragma unroll
-      for (int e = 0; e < 8; ++e) dif_quad<1>(xr[e], xi[e], (c & 1) == 0, twid(tw, 16, e));
+      for (int e = 0; e < 8; ++e) dif_quad<1>(xr[e], xi[e], (c & 1) == 0, w32<T>(2 * e, sgn));
 #pragma unroll
-      for (int e = 0; e < 4; ++e) dif(xr[e], xi[e], xr[e + 4], xi[e + 4], twid(tw, 8, e));
+      for (int e = 0; e < 4; ++e) dif(xr[e], xi[e], xr[e + 4], xi[e + 4], w32<T>(4 * e, sgn));
 #pragma unroll
       for (int e = 0; e < 8; e += 4) {
-        dif(xr[e], xi[e], xr[e + 2], xi[e + 2], twid(tw, 4, 0));
-        dif(xr[e + 1], xi[e + 1], xr[e + 3], xi[e + 3], twid(tw, 4, 1));
+        dif(xr[e], xi[e], xr[e + 2], xi[e + 2], w32<T>(0, sgn));
+        dif(xr[e + 1], xi[e + 1], xr[e + 3], xi[e + 3], w32<T>(8, sgn));
       }
 #pragma unroll
-      for (int e = 0; e < 8; e += 2) dif(xr[e], xi[e], xr[e + 1], xi[e + 1], twid(tw, 2, 0));
+      for (int e = 0; e < 8; e += 2) dif(xr[e], xi[e], xr[e + 1], xi[e + 1], w32<T>(0, sgn));
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         fre[base + e] = xr[e];
@@ -319,6 +337,8 @@ __device__ inline void dif_stage_a(const Acc* ire, const Acc* iim, T* fre, T* fi
 template <typename T, int NT, typename Acc>
 __device__ inline void band_fft_x(Acc* ire, Acc* iim, int rk, int pitch, int log2n0, T inv,
                                   const Cplx<T>* __restrict__ tw) {
+  // sign of the transform: imaginary part of W_4 = exp(sign i pi / 2)
+  const T sgn = twid(tw, 4, 1).y > T(0) ? T(1) : T(-1);
   T* fre = reinterpret_cast<T*>(ire);
   T* fim = reinterpret_cast<T*>(iim);
   const int lb = log2n0 - 5;  // levels of half-span >= 32
@@ -343,9 +363,9 @@ __device__ inline void band_fft_x(Acc* ire, Acc* iim, int rk, int pitch, int log
     __syncthreads();
   }
   if (conv) {
-    dif_stage_a<true, T, NT>(ire, iim, fre, fim, rk, pitch, log2n0, inv, tw);
+    dif_stage_a<true, T, NT>(ire, iim, fre, fim, rk, pitch, log2n0, inv, sgn);
   } else {
-    dif_stage_a<false, T, NT>(ire, iim, fre, fim, rk, pitch, log2n0, inv, tw);
+    dif_stage_a<false, T, NT>(ire, iim, fre, fim, rk, pitch, log2n0, inv, sgn);
   }
   __syncthreads();
 }
