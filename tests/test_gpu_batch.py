@@ -28,6 +28,10 @@ SHAPES = {
     2: [((32, 24), 300), ((8, 8), 0), ((32, 24), 1500), ((60, 2), 77), ((8, 8), 900),
         ((1, 40), 33)],
     3: [((12, 10, 8), 400), ((16, 16, 16), 2500), ((12, 10, 8), 0), ((4, 20, 6), 300)],
+    # 2D with power-of-two fine grids >= 32 per dimension: type 1 runs the
+    # fused path (band spreading + x-FFT in LDS, then y-FFT + deconvolution
+    # in one launch over all transforms)
+    "2f": [((32, 32), 500), ((64, 16), 800), ((32, 32), 0), ((16, 64), 300), ((128, 32), 1200)],
 }
 
 # relative l2 distance allowed between a batched transform and a plain plan
@@ -44,6 +48,7 @@ def _rel(a, b):
 def _setup(dim, prec, seed):
     rng = np.random.default_rng(seed)
     spec = SHAPES[dim]
+    dim = 2 if dim == "2f" else dim
     locs, cs, fs = [], [], []
     for modes, m in spec:
         x = points(rng, dim, m, prec)
@@ -59,17 +64,21 @@ def _to_np(t):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("dim", [1, 2, "2f", 3])
 @pytest.mark.parametrize("order", [nnf.ModeOrder.CMCL, nnf.ModeOrder.FFT])
 @pytest.mark.parametrize("sign", [-1, 1])
 def test_batch_matches_plans(dim, prec, order, sign):
     tol = 1e-5 if prec == "f32" else 1e-9
     opt = nnf.Options(tol=tol, order=order)
-    spec, locs, cs, fs = _setup(dim, prec, 10 * dim + (sign > 0))
+    key = dim
+    dim = 2 if key == "2f" else key
+    spec, locs, cs, fs = _setup(key, prec, 10 * dim + (sign > 0) + (key == "2f"))
     num = [m for _, m in spec]
     modes = [list(md) for md, _ in spec]
     batch = nnf.BatchPlan(opt, sign, num, locs, modes, dtype=RDT[prec], use_graph=False)
     info = batch.info()
+    if key == "2f":
+        assert info["spread_mode"] == "bands"
     assert info["count"] == len(spec)
     assert info["num_points"] == sum(num)
     assert info["num_shapes"] == len({tuple(md) for md, _ in spec})
@@ -105,12 +114,12 @@ def test_batch_matches_plans(dim, prec, order, sign):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_batch_graph_replays(prec):
+@pytest.mark.parametrize("shapes", [2, "2f"])
+def test_batch_graph_replays(prec, shapes):
     """Captured graphs replay correctly when the caller's buffers change
     between calls, with unrelated eager work in between."""
-    dim = 2
     opt = nnf.Options(tol=1e-5 if prec == "f32" else 1e-9)
-    spec, locs, cs, fs = _setup(dim, prec, 99)
+    spec, locs, cs, fs = _setup(shapes, prec, 99)
     num = [m for _, m in spec]
     modes = [list(md) for md, _ in spec]
     gb = nnf.BatchPlan(opt, -1, num, locs, modes, dtype=RDT[prec], use_graph=True)

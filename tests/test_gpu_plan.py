@@ -263,6 +263,11 @@ BAND_T1 = [
     ((64, 48), 3000, 1e-5, -1, 1),
     ((64, 48), 3000, 1e-6, 1, 2),
     ((128, 30), 1000, 1e-9, -1, 0),
+    # power-of-two n1 >= 32 as well: y-FFT + deconvolution fused too
+    ((64, 64), 3000, 1e-5, -1, 0),
+    ((16, 128), 2000, 1e-6, 1, 0),
+    ((256, 16), 4000, 1e-5, 1, 0),
+    ((64, 64), 3000, 1e-10, 1, 0),
 ]
 
 
@@ -303,3 +308,10 @@ def test_type1_spread_modes(case, order, prec):
     same = 1e-12 if prec == "f64" else 5e-6
     assert O.rel_l2(outs["bands"], outs["tiles"]) < same
     assert O.rel_l2(outs["bands_grid"], outs["tiles"]) < same
+    # caller strides: y modes fastest (transposed output)
+    plan = nnf.Plan(opt, sign, m, loc, modes, spread_mode="bands", band_rows=rows)
+    out_t = torch.zeros(int(np.prod(modes)), dtype=CDT[prec], device="cuda")
+    plan.transform_type_1(cin, out_t, [modes[1], 1])
+    torch.cuda.synchronize()
+    got_t = out_t.cpu().numpy().astype(np.complex128).reshape(modes[0], modes[1]).T.ravel()
+    assert O.rel_l2(got_t, outs["bands"]) < same
