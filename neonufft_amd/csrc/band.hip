@@ -20,7 +20,6 @@ namespace {
 
 constexpr int kRowBlock = 256;
 constexpr int kGatherItems = 8;
-constexpr int kBoundBlocks = 64;
 // Band workgroup size and points per thread per pass: every thread issues the
 // loads of kBandBatch points before using them, so 16 waves per CU (fp32) keep
 // several loads in flight each while the LDS atomics of earlier points run.
@@ -73,6 +72,65 @@ __global__ void __launch_bounds__(kRowBlock)
   }
 }
 
+// Column blocks of the fixed-point bound: n0 split into at most kBoundBlocks
+// blocks of bw cells.
+__host__ __device__ inline int bound_block_width(int n0) {
+  return (n0 + kBoundBlocks - 1) / kBoundBlocks > 1 ? (n0 + kBoundBlocks - 1) / kBoundBlocks : 1;
+}
+
+// bcount[row * kBoundBlocks + k] = view points of first kernel row `row` whose
+// kernel x-window meets column block k.
+template <typename T>
+__global__ void __launch_bounds__(kRowBlock)
+    row_block_count_kernel(const PointRec<T, 2>* __restrict__ view, std::size_t m, int n0, int ns,
+                           uint32_t* __restrict__ bcount) {
+  const int bw = bound_block_width(n0);
+  const int nb = (n0 + bw - 1) / bw;
+  for (std::size_t p = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kRowBlock; p < m;
+       p += static_cast<std::size_t>(gridDim.x) * kRowBlock) {
+    const uint32_t cell = view[p].cell;
+    const int row = static_cast<int>(cell >> CellPack<2>::bits);
+    const int x0 = static_cast<int>(cell & CellPack<2>::mask) - ns / 2;
+    const int xa = x0 < 0 ? x0 + n0 : x0;
+    const int xe = x0 + ns - 1;
+    const int xb = xe >= n0 ? xe - n0 : xe;
+    const int kb = xb / bw;
+    uint32_t* c = bcount + static_cast<std::size_t>(row) * kBoundBlocks;
+    for (int b = xa / bw;; b = (b + 1 == nb ? 0 : b + 1)) {
+      atomicAdd(&c[b], 1u);
+      if (b == kb) break;
+    }
+  }
+}
+
+__global__ void clear_kernel(unsigned long long* __restrict__ p, int n) {
+  const int i = threadIdx.x + blockIdx.x * blockDim.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// Largest |Re c| + |Im c| of a job, as the bits of a non-negative T (whose
+// unsigned order is the numeric order).
+__device__ inline void max_into(void* slot, float v) {
+  atomicMax(static_cast<unsigned int*>(slot), __float_as_uint(v));
+}
+__device__ inline void max_into(void* slot, double v) {
+  atomicMax(static_cast<unsigned long long*>(slot),
+            static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+// Slot k of a job's kMaxSlots maxima (each 8 bytes).
+__device__ inline void* max_slot(void* base, int k, float) {
+  return static_cast<unsigned long long*>(base) + k;
+}
+__device__ inline void* max_slot(void* base, int k, double) {
+  return static_cast<unsigned long long*>(base) + k;
+}
+__device__ inline float max_from(const void* slot, float) {
+  return __uint_as_float(*static_cast<const unsigned int*>(slot));
+}
+__device__ inline double max_from(const void* slot, double) {
+  return __longlong_as_double(static_cast<long long>(*static_cast<const unsigned long long*>(slot)));
+}
+
 // Index of the job owning flattened item `i` (jobs sorted by `off`).
 template <typename T>
 __device__ inline int find_job(const BandJob<T>* jobs, int njobs, std::size_t i, bool by_band) {
@@ -89,36 +147,66 @@ __device__ inline int find_job(const BandJob<T>* jobs, int njobs, std::size_t i,
   return lo;
 }
 
-// sorted[p] = in[perm[p]] over the points of every job (flattened).
+// sorted[p] = in[perm[p]] over the points of every job (flattened), and the
+// largest |Re c| + |Im c| of every job into its slot (cleared beforehand):
+// one atomic per wave when all of the wave's items belong to one job (the
+// common case), else one per item.
 template <typename T>
 __global__ void __launch_bounds__(kRowBlock)
     band_gather_kernel(const BandJob<T>* __restrict__ jobs, BandJob<T> single, int njobs,
                        std::size_t total) {
   const std::size_t base = static_cast<std::size_t>(blockIdx.x) * (kRowBlock * kGatherItems);
   if (base >= total) return;
-  int j = jobs ? find_job(jobs, njobs, base, false) : 0;
+  const int j0 = jobs ? find_job(jobs, njobs, base, false) : 0;
+  int j = j0;
   BandJob<T> job = jobs ? jobs[j] : single;
   // all index loads, then all strength loads, then the stores
   Cplx<T>* dst[kGatherItems];
   const Cplx<T>* src[kGatherItems];
+  int jk[kGatherItems];
 #pragma unroll
   for (int k = 0; k < kGatherItems; ++k) {
     const std::size_t i = base + threadIdx.x + static_cast<std::size_t>(k) * kRowBlock;
     dst[k] = nullptr;
     src[k] = nullptr;
+    jk[k] = j0;
     if (i < total) {
       while (i >= job.pt_off + job.m) job = jobs[++j];
       const std::size_t p = i - job.pt_off;
       dst[k] = job.sorted + p;
       src[k] = job.in + job.perm[p];
+      jk[k] = j;
     }
   }
   Cplx<T> v[kGatherItems];
 #pragma unroll
   for (int k = 0; k < kGatherItems; ++k) v[k] = src[k] ? *src[k] : Cplx<T>{T(0), T(0)};
+  T mx = T(0);
+  bool mixed = false;
 #pragma unroll
   for (int k = 0; k < kGatherItems; ++k) {
-    if (dst[k]) *dst[k] = v[k];
+    if (dst[k]) {
+      *dst[k] = v[k];
+      mx = max(mx, fabs(v[k].x) + fabs(v[k].y));
+      mixed = mixed || jk[k] != j0;
+    }
+  }
+  // a job's maximum is spread over kMaxSlots slots (slot = workgroup mod
+  // kMaxSlots) so that its many workgroups do not queue on one address; no
+  // barrier here, which would also wait for the stores above
+  const int slot = static_cast<int>(blockIdx.x % kMaxSlots);
+  if (__ballot(mixed) == 0) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) max_into(max_slot(jobs ? jobs[j0].cmax : single.cmax, slot, mx), mx);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kGatherItems; ++k) {
+      if (dst[k]) {
+        const T a = fabs(v[k].x) + fabs(v[k].y);
+        max_into(max_slot(jobs[jk[k]].cmax, slot, a), a);
+      }
+    }
   }
 }
 
@@ -372,8 +460,9 @@ __device__ inline void band_fft_x(Acc* ire, Acc* iim, int rk, int pitch, int log
 
 // One band of rows [y0, y0 + rk) of one job.
 //  1. bound: every cell value is at most (points whose kernel x-window meets
-//     the cell's 1/64 column block) x max |Re c| + |Im c|; the largest block
-//     count times the largest strength fixes the fixed-point scale;
+//     the cell's 1/64 column block: counts per row precomputed with the row
+//     view) x (the job's largest |Re c| + |Im c|, from the gather); this fixes
+//     the fixed-point scale;
 //  2. accumulation with integer LDS atomics (rows outside the band skipped;
 //     a row no lane of a wave needs is skipped by the whole wave);
 //  3. each cell converted back once and written (or added) to the grid, or
@@ -387,8 +476,6 @@ __global__ void __launch_bounds__(kBandThreads<T>)
   constexpr int NT = kBandThreads<T>;
   constexpr int KB = kBandBatch;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ T wred[NT / 64];
-  __shared__ uint32_t blk[kBoundBlocks];
 
   const BandJob<T> job = jobs ? jobs[find_job(jobs, njobs, blockIdx.x, true)] : single;
   const int n0 = job.n0, n1 = job.n1, pitch = job.pitch;
@@ -397,7 +484,6 @@ __global__ void __launch_bounds__(kBandThreads<T>)
   Acc* tre = reinterpret_cast<Acc*>(smem_raw);
   Acc* tim = tre + rk * pitch;
   for (int i = threadIdx.x; i < 2 * rk * pitch; i += NT) tre[i] = Acc(0);
-  if (threadIdx.x < kBoundBlocks) blk[threadIdx.x] = 0u;
 
   // Points whose first kernel row lies in [y0 - NS + 1, y0 + rk - 1] (mod
   // n1): one range of the row-sorted view, or two when it wraps below row 0
@@ -420,49 +506,30 @@ __global__ void __launch_bounds__(kBandThreads<T>)
   const PointRec<T, 2>* __restrict__ pts = job.pts;
   const Cplx<T>* __restrict__ str = job.sorted;
 
-  const int bw = max(1, (n0 + kBoundBlocks - 1) / kBoundBlocks);
-  const int nb = (n0 + bw - 1) / bw;
-  __syncthreads();  // zeroed counters before the bound pass
-
-  T cm = T(0);
-  for (uint32_t q0 = 0; q0 < total; q0 += NT * KB) {
-    uint32_t cell[KB];
-    Cplx<T> c[KB];
-#pragma unroll
-    for (int k = 0; k < KB; ++k) {
-      const uint32_t q = q0 + threadIdx.x + k * NT;
-      cell[k] = 0xffffffffu;
-      c[k] = Cplx<T>{T(0), T(0)};
-      if (q < total) {
-        const uint32_t p = q < cnt0 ? b0 + q : b1 + (q - cnt0);
-        cell[k] = pts[p].cell;
-        c[k] = str[p];
-      }
+  // Fixed-point bound: a cell's value is at most (points whose kernel
+  // x-window meets its column block) x the job's largest |Re c| + |Im c|.
+  // Lane k of wave w sums block k's counts over rows w, w + NW, ... of the
+  // band's point rows; after the barrier every wave adds the NW partial sums
+  // and takes the maxima (of the blocks and of the job's maximum slots).
+  constexpr int NW = NT / 64;
+  __shared__ uint32_t part[NW][kBoundBlocks];
+  {
+    const int k = static_cast<int>(threadIdx.x & 63);
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    uint32_t bc = 0;
+#pragma unroll 4
+    for (int rr = lo_row + w; rr < y0 + rk; rr += NW) {
+      const int row = rr < 0 ? rr + n1 : rr;
+      bc += job.bcount[static_cast<std::size_t>(row) * kBoundBlocks + k];
     }
-#pragma unroll
-    for (int k = 0; k < KB; ++k) {
-      if (cell[k] == 0xffffffffu) continue;
-      cm = max(cm, fabs(c[k].x) + fabs(c[k].y));
-      const int x0 = static_cast<int>(cell[k] & CellPack<2>::mask) - NS / 2;
-      const int xa = x0 < 0 ? x0 + n0 : x0;
-      const int xe = x0 + NS - 1;
-      const int xb = xe >= n0 ? xe - n0 : xe;
-      const int kb = xb / bw;
-      for (int b = xa / bw;; b = (b + 1 == nb ? 0 : b + 1)) {
-        atomicAdd(&blk[b], 1u);
-        if (b == kb) break;
-      }
-    }
+    part[w][k] = bc;
   }
-  cm = wave_max(cm);
-  if ((threadIdx.x & 63) == 0) wred[threadIdx.x >> 6] = cm;
-  __syncthreads();
-  T cmax = T(0);
+  const T cm = max_from(static_cast<const unsigned long long*>(job.cmax) + (threadIdx.x & 63), T(0));
+  __syncthreads();  // zeroed LDS and the partial counts
+  uint32_t bc = 0;
 #pragma unroll
-  for (int w = 0; w < NT / 64; ++w) cmax = max(cmax, wred[w]);
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t bmax = wave_max(lane < static_cast<uint32_t>(nb) ? blk[lane] : 0u);
-  const T bound = static_cast<T>(bmax) * cmax;
+  for (int w = 0; w < NW; ++w) bc += part[w][threadIdx.x & 63];
+  const T bound = static_cast<T>(wave_max(bc)) * wave_max(cm);
   const T one = static_cast<T>(1ull << FixedAcc<T>::bits);
   const T scale = bound > T(0) ? one / bound : T(0);
 
@@ -615,6 +682,21 @@ void launch_row_scatter(const PointRec<T, 2>* recs, const uint32_t* perm, std::s
 }
 
 template <typename T>
+void launch_row_block_count(const PointRec<T, 2>* view, std::size_t m, int n0, int ns,
+                            uint32_t* bcount, hipStream_t stream) {
+  if (m == 0) return;
+  hipLaunchKernelGGL((row_block_count_kernel<T>), dim3(blocks_for(m, kRowBlock)),
+                     dim3(kRowBlock), 0, stream, view, m, n0, ns, bcount);
+  check_launch("row_block_count_kernel");
+}
+
+void launch_clear_max(unsigned long long* slots, int n, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(clear_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, slots, n);
+  check_launch("clear_kernel");
+}
+
+template <typename T>
 void launch_band_gather(const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
                         std::size_t total_points, hipStream_t stream) {
   if (total_points == 0) return;
@@ -672,6 +754,8 @@ std::size_t band_lds_bytes(int rows, int pitch, std::size_t acc_bytes) {
   template void launch_row_scatter<T>(const PointRec<T, 2>*, const uint32_t*, std::size_t,      \
                                       const BinGeom&, int, const uint32_t*, const uint32_t*,    \
                                       PointRec<T, 2>*, uint32_t*, hipStream_t);                 \
+  template void launch_row_block_count<T>(const PointRec<T, 2>*, std::size_t, int, int,          \
+                                          uint32_t*, hipStream_t);                               \
   template void launch_band_gather<T>(const BandJob<T>*, const BandJob<T>&, int, std::size_t,   \
                                       hipStream_t);                                             \
   template void launch_band_spread<T>(int, const BandJob<T>*, const BandJob<T>&, int,           \

@@ -183,6 +183,7 @@ public:
     host_io_.resize(count_);
     up_sjobs_.init(sizeof(SpreadJob<T, DIM>) * count_);
     if (bands_) {
+      cmax_.reset(static_cast<std::size_t>(count_) * kMaxSlots, alloc_);
       bjobs_.reset(count_, alloc_);
       host_bjobs_.resize(count_);
       rows_.resize(count_);
@@ -268,6 +269,8 @@ public:
           j.pts = rows_[t].recs();
           j.perm = rows_[t].perm();
           j.row_start = rows_[t].row_start();
+          j.bcount = rows_[t].block_counts();
+          j.cmax = cmax_.data() + static_cast<std::size_t>(t) * kMaxSlots;
           j.in = in[t];
           j.sorted = sorted_.data() + it.pt_off;
           j.grid = grids_.data() + it.grid_off;
@@ -372,6 +375,7 @@ private:
       // every fine-grid cell is written, so the grids need no zero fill
       if (bands_) {
         if constexpr (DIM == 2) {
+          launch_clear_max(cmax_.data(), count_ * kMaxSlots, s);
           launch_band_gather<T>(bjobs_.data(), BandJob<T>{}, count_, total_points_, s);
           launch_band_spread<T>(ks_.ns, bjobs_.data(), BandJob<T>{}, count_, total_bands_,
                                 band_lds_, false, fused_, ks_.es, s);
@@ -497,6 +501,7 @@ private:
   bool fused_ = true;    // ... with the x-FFT inside the band kernel
   IntType tw_len_ = 1;
   DeviceBuffer<Cplx<T>> tw_;
+  DeviceBuffer<unsigned long long> cmax_;  // per transform: largest strength
   std::size_t total_bands_ = 0, band_lds_ = 0;
   std::vector<RowView<T>> rows_;  // row-sorted views (row bands)
   DeviceBuffer<Cplx<T>> sorted_;  // strengths of all transforms in view order

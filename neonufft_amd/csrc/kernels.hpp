@@ -181,6 +181,8 @@ struct BandJob {
   const PointRec<T, 2>* pts;   // row-sorted records
   const uint32_t* perm;        // row-sorted position -> caller's point index
   const uint32_t* row_start;   // n1 + 1 offsets into the view
+  const uint32_t* bcount;      // n1 x kBoundBlocks: points per (first row, column block)
+  void* cmax;                  // kMaxSlots x 8 bytes: largest |Re c| + |Im c| (bits of T)
   const Cplx<T>* in;           // caller's strengths
   Cplx<T>* sorted;             // strengths in view order (gather output)
   Cplx<T>* grid;               // fine grid, x fastest
@@ -195,9 +197,21 @@ struct BandJob {
   int tw_len, N0, log2n0, log2rows;
 };
 
-// Strengths into view order (sorted[p] = in[perm[p]]) for every job; `jobs`
-// is a device table of njobs entries sorted by pt_off, or null to use
-// `single`. total_points = sum of m.
+// Points of each first kernel row whose kernel x-window meets each of the
+// kBoundBlocks column blocks of width max(1, ceil(n0 / kBoundBlocks)).
+constexpr int kBoundBlocks = 64;
+template <typename T>
+void launch_row_block_count(const PointRec<T, 2>* view, std::size_t m, int n0, int ns,
+                            uint32_t* bcount, hipStream_t stream);
+// Zeroes n strength-maximum slots (a kernel, so graphs may record it). Each
+// job owns kMaxSlots consecutive slots (its maximum is their maximum).
+constexpr int kMaxSlots = 64;
+void launch_clear_max(unsigned long long* slots, int n, hipStream_t stream);
+
+// Strengths into view order (sorted[p] = in[perm[p]]) for every job, and
+// each job's largest |Re c| + |Im c| into its (cleared) cmax slot; `jobs` is
+// a device table of njobs entries sorted by pt_off, or null to use `single`.
+// total_points = sum of m.
 template <typename T>
 void launch_band_gather(const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
                         std::size_t total_points, hipStream_t stream);

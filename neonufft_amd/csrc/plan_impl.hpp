@@ -135,6 +135,7 @@ public:
     if constexpr (DIM == 2) {
       band_ = plan_band_layout(n_, ks_.ns, sizeof(T), tune_, 256);
       band_rows_ = band_.rows;
+      if (band_rows_ > 0) cmax_.reset(kMaxSlots, alloc_);
       if (band_.fft) upload_twiddles<T>(tw_, n_[0], sign_, stream_, alloc_);
     }
     set_points(num_nu, loc);
@@ -174,6 +175,7 @@ public:
     if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
     if constexpr (DIM == 2) {
       if (band_rows_ > 0) {
+        launch_clear_max(cmax_.data(), kMaxSlots, stream_);
         launch_band_gather<T>(nullptr, band_job(in), 1, pts_.size(), stream_);
         return;
       }
@@ -211,6 +213,7 @@ public:
         // gather, then spread + x-FFT per band straight into the pruned
         // FFT's intermediate; y-FFT and deconvolution follow
         if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
+        launch_clear_max(cmax_.data(), kMaxSlots, stream_);
         launch_band_gather<T>(nullptr, band_job(in), 1, pts_.size(), stream_);
         const int nb = static_cast<int>((n_[1] + band_rows_ - 1) / band_rows_);
         launch_band_spread<T>(ks_.ns, nullptr, band_job(nullptr), 1, static_cast<std::size_t>(nb),
@@ -274,6 +277,8 @@ private:
     j.pts = rows_.recs();
     j.perm = rows_.perm();
     j.row_start = rows_.row_start();
+    j.bcount = rows_.block_counts();
+    j.cmax = cmax_.data();
     j.in = in;
     j.sorted = sorted_.data();
     j.grid = grid_.data();
@@ -305,6 +310,7 @@ private:
   int band_rows_ = 0;             // 2D row-band spreading (0 = subgrid tiles)
   BandLayout band_;
   DeviceBuffer<Cplx<T>> tw_;      // x-FFT twiddles of the fused type 1
+  DeviceBuffer<unsigned long long> cmax_;  // largest strength (row bands)
   RowView<T> rows_;               // row-sorted view (row bands)
   DeviceBuffer<Cplx<T>> sorted_;  // strengths in view order (row bands)
   DeviceBuffer<Cplx<T>> grid_;

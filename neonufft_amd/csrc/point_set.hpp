@@ -127,17 +127,22 @@ public:
     og.tile[1] = 0;
     og.tile[2] = 1;
     launch_bank_order<T, 2>(chunks_.data(), max_chunks, recs_.data(), perm_.data(), og, stream);
+    bcount_.ensure(n1 * kBoundBlocks, alloc);
+    check_hip(hipMemsetAsync(bcount_.data(), 0, n1 * kBoundBlocks * sizeof(uint32_t), stream),
+              "hipMemsetAsync(block counts)");
+    launch_row_block_count<T>(recs_.data(), m, g.n[0], ns, bcount_.data(), stream);
   }
 
   const PointRec<T, 2>* recs() const { return recs_.data(); }
   const uint32_t* perm() const { return perm_.data(); }
   const uint32_t* row_start() const { return start_.data(); }
+  const uint32_t* block_counts() const { return bcount_.data(); }
   std::size_t size() const { return m_; }
 
 private:
   std::size_t m_ = 0;
   DeviceBuffer<PointRec<T, 2>> recs_;
-  DeviceBuffer<uint32_t> perm_, rank_, count_, start_, nchunk_, chunk_start_, scratch_;
+  DeviceBuffer<uint32_t> perm_, rank_, count_, start_, nchunk_, chunk_start_, scratch_, bcount_;
   DeviceBuffer<Subprob> chunks_;
 };
 
