@@ -52,9 +52,9 @@ BinGeom choose_bins(int dim, const IntType* n, int ns, std::size_t elem_bytes,
 int band_rows(IntType n0, IntType pitch, IntType n1, int ns, std::size_t acc_bytes,
               std::size_t budget, int min_bands);
 
-// LDS bytes of one row band (8 rows of a 2048-cell fp32 grid with the fused
-// FFT's padded pitch fit).
-constexpr std::size_t kBandBudget = 136 * 1024;
+// LDS bytes of one row band: two bands per CU (4 rows of a 2048-cell fp32
+// grid with the fused FFT's padded pitch fit).
+constexpr std::size_t kBandBudget = 68 * 1024;
 
 // Default number of points per subproblem.
 constexpr uint32_t kDefaultMaxPoints = 2048;

@@ -154,11 +154,11 @@ def test_band_spread(case, prec):
     fused = n0 >= 32 and (n0 & (n0 - 1)) == 0
     if rows == -1:
         rows = n1 - ns + 1
-    # requested rows, capped by the 136 KiB LDS band (two int32 / int64
+    # requested rows, capped by the 68 KiB LDS band (two int32 / int64
     # accumulator planes per row of n0 words, n0 + 8 with the fused x-FFT)
     # and rounded down to a power of two with the fused x-FFT
     pitch = n0 + 8 if fused else n0
-    expect = min(rows, n1 - ns + 1, (136 * 1024) // (2 * pitch * (4 if prec == "f32" else 8)))
+    expect = min(rows, n1 - ns + 1, (68 * 1024) // (2 * pitch * (4 if prec == "f32" else 8)))
     if fused and expect > 0:
         expect = 1 << (expect.bit_length() - 1)
     rng, plan, info, u = _plan(prec, 2, modes, m, 5, spread_mode="bands", band_rows=rows)
