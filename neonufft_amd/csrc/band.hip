@@ -384,6 +384,29 @@ __device__ inline void dif2_pass32(const Acc* ire, const Acc* iim, T* fre, T* fi
   }
 }
 
+// 8 consecutive LDS elements of a 16-byte aligned address as 16-byte accesses
+// (lanes 32 bytes apart then meet each bank at most twice per wave; 4-byte
+// accesses at that stride would queue 16 lanes on each of 4 banks).
+template <typename E>
+__device__ inline void lds_load8(const E* p, E (&v)[8]) {
+  constexpr int kVec = 16 / sizeof(E);
+#pragma unroll
+  for (int i = 0; i < 8; i += kVec) {
+    const uint4 u = *reinterpret_cast<const uint4*>(p + i);
+    __builtin_memcpy(&v[i], &u, 16);
+  }
+}
+template <typename E>
+__device__ inline void lds_store8(E* p, const E (&v)[8]) {
+  constexpr int kVec = 16 / sizeof(E);
+#pragma unroll
+  for (int i = 0; i < 8; i += kVec) {
+    uint4 u;
+    __builtin_memcpy(&u, &v[i], 16);
+    *reinterpret_cast<uint4*>(p + i) = u;
+  }
+}
+
 template <bool CONV, typename T, int NT, typename Acc>
 __device__ inline void dif_stage_a(const Acc* ire, const Acc* iim, T* fre, T* fim, int rk,
                                    int pitch, int log2n0, T inv, T sgn) {
@@ -396,12 +419,20 @@ __device__ inline void dif_stage_a(const Acc* ire, const Acc* iim, T* fre, T* fi
                                      : static_cast<int>(threadIdx.x) & (chunks - 1));
     const int r0 = chunks >= NT ? 0 : static_cast<int>(threadIdx.x) >> lc;
     for (int r = r0; r < rk; r += rstep) {
-      const int base = r * pitch + 8 * k;
+      const int base = r * pitch + 8 * k;  // pitch is a multiple of 8
       T xr[8], xi[8];
+      if constexpr (CONV) {
+        Acc ar[8], ai[8];
+        lds_load8(ire + base, ar);
+        lds_load8(iim + base, ai);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        xr[e] = ldv<CONV>(ire, fre, base + e, inv);
-        xi[e] = ldv<CONV>(iim, fim, base + e, inv);
+        for (int e = 0; e < 8; ++e) {
+          xr[e] = static_cast<T>(ar[e]) * inv;
+          xi[e] = static_cast<T>(ai[e]) * inv;
+        }
+      } else {
+        lds_load8(fre + base, xr);
+        lds_load8(fim + base, xi);
       }
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
@@ -419,11 +450,8 @@ __device__ inline void dif_stage_a(const Acc* ire, const Acc* iim, T* fre, T* fi
       }
 #pragma unroll
       for (int e = 0; e < 8; e += 2) dif(xr[e], xi[e], xr[e + 1], xi[e + 1], w32<T>(0, sgn));
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        fre[base + e] = xr[e];
-        fim[base + e] = xi[e];
-      }
+      lds_store8(fre + base, xr);
+      lds_store8(fim + base, xi);
     }
   }
 }
