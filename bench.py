@@ -164,21 +164,29 @@ def batched_leg(args, dev, stream):
         a, b = outs_b[t], outs_p[t]
         max_diff = max(max_diff, float(torch.linalg.vector_norm(a - b)
                                        / torch.linalg.vector_norm(b)))
-    # FP64 direct sum on 1000 sampled modes of the transform with fewest points
+    def direct(t, samples):
+        """Relative l2 error of batched transform t on sampled modes against
+        the FP64 direct sum."""
+        n = int(nmodes[t])
+        x = locs[t][0].double().cpu().numpy()
+        y = locs[t][1].double().cpu().numpy()
+        c = cs[t].cpu().numpy().astype(np.complex128)
+        idx = np.sort(rng.choice(n * n, size=samples, replace=False))
+        kx = (idx % n - n // 2).astype(np.float64)
+        ky = (idx // n - n // 2).astype(np.float64)
+        ref = np.zeros(len(idx), dtype=np.complex128)
+        for j0 in range(0, len(x), 4096):
+            sl = slice(j0, j0 + 4096)
+            ref += np.exp(-1j * (np.outer(kx, x[sl]) + np.outer(ky, y[sl]))) @ c[sl]
+        got = outs_b[t].cpu().numpy()[idx].astype(np.complex128)
+        return float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+    # the transform with fewest points (1000 modes) and the one with most
+    # points per fine-grid cell (100 modes: the fixed-point sums are densest)
     t = int(np.argmin(ms))
-    n = int(nmodes[t])
-    x = locs[t][0].double().cpu().numpy()
-    y = locs[t][1].double().cpu().numpy()
-    c = cs[t].cpu().numpy().astype(np.complex128)
-    idx = np.sort(rng.choice(n * n, size=1000, replace=False))
-    kx = (idx % n - n // 2).astype(np.float64)
-    ky = (idx // n - n // 2).astype(np.float64)
-    ref = np.zeros(len(idx), dtype=np.complex128)
-    for j0 in range(0, len(x), 4096):
-        sl = slice(j0, j0 + 4096)
-        ref += np.exp(-1j * (np.outer(kx, x[sl]) + np.outer(ky, y[sl]))) @ c[sl]
-    got = outs_b[t].cpu().numpy()[idx].astype(np.complex128)
-    direct_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    direct_err = direct(t, 1000)
+    t_dense = int(np.argmax(ms / (nmodes.astype(np.float64) ** 2)))
+    direct_err_dense = direct(t_dense, 100)
 
     total = int(ms.sum())
     out = {
@@ -199,7 +207,11 @@ def batched_leg(args, dev, stream):
         "batch_create_ms": batch_create_ms,
         "max_rel_diff_vs_plain": max_diff,
         "direct_rel_l2": direct_err,
-        "direct_check": f"1000 sampled modes of transform {t} (M={int(ms[t])}, {n}^2 modes)",
+        "direct_rel_l2_densest": direct_err_dense,
+        "direct_check_densest": (f"100 sampled modes of transform {t_dense} "
+                                 f"(M={int(ms[t_dense])}, {int(nmodes[t_dense])}^2 modes)"),
+        "direct_check": (f"1000 sampled modes of transform {t} (M={int(ms[t])}, "
+                         f"{int(nmodes[t])}^2 modes)"),
     }
     del plans, batch
     torch.cuda.synchronize()

@@ -109,6 +109,22 @@ __global__ void __launch_bounds__(kRowBlock)
   }
 }
 
+// *out = max over rows r and blocks k of sum_{j < ns} bcount[r - j][k]
+// (rows mod n1): the largest point count any cell's bound can see.
+__global__ void __launch_bounds__(kRowBlock)
+    row_window_max_kernel(const uint32_t* __restrict__ bcount, int n1, int ns,
+                          uint32_t* __restrict__ out) {
+  const int i = threadIdx.x + blockIdx.x * kRowBlock;
+  if (i >= n1 * kBoundBlocks) return;
+  const int r = i / kBoundBlocks, k = i - r * kBoundBlocks;
+  uint32_t s = 0;
+  for (int j = 0; j < ns; ++j) {
+    const int row = r - j < 0 ? r - j + n1 : r - j;
+    s += bcount[static_cast<std::size_t>(row) * kBoundBlocks + k];
+  }
+  atomicMax(out, s);
+}
+
 __global__ void clear_kernel(unsigned long long* __restrict__ p, int n) {
   const int i = threadIdx.x + blockIdx.x * blockDim.x;
   if (i < n) p[i] = 0ull;
@@ -493,10 +509,10 @@ __device__ inline void band_fft_x(Acc* ire, Acc* iim, int rk, int pitch, int log
 }
 
 // One band of rows [y0, y0 + rk) of one job.
-//  1. bound: every cell value is at most (points whose kernel x-window meets
-//     the cell's 1/64 column block: counts per row precomputed with the row
-//     view) x (the job's largest |Re c| + |Im c|, from the gather); this fixes
-//     the fixed-point scale;
+//  1. bound: every cell value is at most (points of the cell row's NS-row
+//     window whose kernel x-window meets the cell's 1/64 column block: counts
+//     per row precomputed with the row view) x (the job's largest
+//     |Re c| + |Im c|, from the gather); this fixes the fixed-point scale;
 //  2. accumulation with integer LDS atomics (rows outside the band skipped;
 //     a row no lane of a wave needs is skipped by the whole wave);
 //  3. each cell converted back once and written (or added) to the grid, or
@@ -540,29 +556,35 @@ __global__ void __launch_bounds__(kBandThreads<T>)
   const PointRec<T, 2>* __restrict__ pts = job.pts;
   const Cplx<T>* __restrict__ str = job.sorted;
 
-  // Fixed-point bound: a cell's value is at most (points whose kernel
-  // x-window meets its column block) x the job's largest |Re c| + |Im c|.
-  // Lane k of wave w sums block k's counts over rows w, w + NW, ... of the
-  // band's point rows; after the barrier every wave adds the NW partial sums
-  // and takes the maxima (of the blocks and of the job's maximum slots).
+  // Fixed-point bound: cell (r, x) only receives points whose first kernel
+  // row lies in [r - NS + 1, r] and whose kernel x-window meets x's column
+  // block, each at most the job's largest |Re c| + |Im c|. Lane k of wave w
+  // takes the band rows w, w + NW, ... and the maximum over them of block k's
+  // counts summed over that NS-row window; after the barrier every wave takes
+  // the maxima over waves and blocks (and of the job's maximum slots).
   constexpr int NW = NT / 64;
   __shared__ uint32_t part[NW][kBoundBlocks];
   {
     const int k = static_cast<int>(threadIdx.x & 63);
     const int w = static_cast<int>(threadIdx.x >> 6);
-    uint32_t bc = 0;
-#pragma unroll 4
-    for (int rr = lo_row + w; rr < y0 + rk; rr += NW) {
-      const int row = rr < 0 ? rr + n1 : rr;
-      bc += job.bcount[static_cast<std::size_t>(row) * kBoundBlocks + k];
+    uint32_t bm = 0;
+    for (int r = w; r < rk; r += NW) {
+      uint32_t bc = 0;
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        int row = y0 + r - j;
+        row = row < 0 ? row + n1 : row;
+        bc += job.bcount[static_cast<std::size_t>(row) * kBoundBlocks + k];
+      }
+      bm = max(bm, bc);
     }
-    part[w][k] = bc;
+    part[w][k] = bm;
   }
   const T cm = max_from(static_cast<const unsigned long long*>(job.cmax) + (threadIdx.x & 63), T(0));
-  __syncthreads();  // zeroed LDS and the partial counts
+  __syncthreads();  // zeroed LDS and the partial maxima
   uint32_t bc = 0;
 #pragma unroll
-  for (int w = 0; w < NW; ++w) bc += part[w][threadIdx.x & 63];
+  for (int w = 0; w < NW; ++w) bc = max(bc, part[w][threadIdx.x & 63]);
   const T bound = static_cast<T>(wave_max(bc)) * wave_max(cm);
   const T one = static_cast<T>(1ull << FixedAcc<T>::bits);
   const T scale = bound > T(0) ? one / bound : T(0);
@@ -722,6 +744,13 @@ void launch_row_block_count(const PointRec<T, 2>* view, std::size_t m, int n0, i
   hipLaunchKernelGGL((row_block_count_kernel<T>), dim3(blocks_for(m, kRowBlock)),
                      dim3(kRowBlock), 0, stream, view, m, n0, ns, bcount);
   check_launch("row_block_count_kernel");
+}
+
+void launch_row_window_max(const uint32_t* bcount, int n1, int ns, uint32_t* out,
+                           hipStream_t stream) {
+  hipLaunchKernelGGL(row_window_max_kernel, dim3(blocks_for(static_cast<std::size_t>(n1) * kBoundBlocks, kRowBlock)),
+                     dim3(kRowBlock), 0, stream, bcount, n1, ns, out);
+  check_launch("row_window_max_kernel");
 }
 
 void launch_clear_max(unsigned long long* slots, int n, hipStream_t stream) {

@@ -217,26 +217,48 @@ public:
       pts_[t].build(l, static_cast<std::size_t>(num_nu[t]), fp, groups_[items_[t].group].geom,
                     ks_.ns, tune_.max_points, stream_, alloc_);
       items_[t].sub_off = total_sub;
-      items_[t].tiles_off = total_tiles;
       items_[t].pt_off = total_points_;
       total_sub += pts_[t].nsub();
-      if (!bands_) total_tiles += pts_[t].nsub() * tile_cells(groups_[items_[t].group].geom);
       total_points_ += static_cast<std::size_t>(num_nu[t]);
     }
-    tiles_.ensure(total_tiles ? total_tiles : 1, alloc_);
+    bands_on_ = false;
     if constexpr (DIM == 2) {
       if (bands_) {
         total_bands_ = 0;
+        row_host_.resize(count_);
         for (int t = 0; t < count_; ++t) {
           const Group& g = groups_[items_[t].group];
           rows_[t].build(pts_[t], ks_.ns, stream_, alloc_);
           items_[t].band_off = total_bands_;
           total_bands_ += static_cast<std::size_t>((g.n[1] + g.band.rows - 1) / g.band.rows);
+          row_host_[t].resize(static_cast<std::size_t>(g.n[1]) + 2);
+          check_hip(hipMemcpyAsync(row_host_[t].data(), rows_[t].row_start(),
+                                   (row_host_[t].size() - 1) * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, stream_),
+                    "hipMemcpyAsync(row offsets)");
+          check_hip(hipMemcpyAsync(row_host_[t].data() + g.n[1] + 1, rows_[t].window_max(),
+                                   sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+                    "hipMemcpyAsync(window max)");
         }
         sorted_.ensure(total_points_ ? total_points_ : 1, alloc_);
         up_bjobs_.invalidate();
+        check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        // every transform falls back to tiles when one of them does not
+        // suit row bands
+        bands_on_ = true;
+        for (int t = 0; t < count_ && bands_on_; ++t) {
+          const Group& g = groups_[items_[t].group];
+          bands_on_ = bands_suitable(row_host_[t], static_cast<int>(g.n[1]), g.band.rows, ks_.ns,
+                                     row_host_[t][static_cast<std::size_t>(g.n[1]) + 1],
+                                     tune_.spread_mode == kSpreadAuto);
+        }
       }
     }
+    for (int t = 0; t < count_; ++t) {
+      items_[t].tiles_off = total_tiles;
+      if (!bands_on_) total_tiles += pts_[t].nsub() * tile_cells(groups_[items_[t].group].geom);
+    }
+    tiles_.ensure(total_tiles ? total_tiles : 1, alloc_);
     for (int t = 0; t < count_; ++t) {
       const Item& it = items_[t];
       const Group& g = groups_[it.group];
@@ -261,7 +283,7 @@ public:
   void transform_type_1(const Cplx<T>* const* in, Cplx<T>* const* out) {
     if (!in || !out) throw neonufft::InputError("Neonufft:: null pointer table");
     if constexpr (DIM == 2) {
-      if (bands_) {
+      if (bands_on_) {
         for (int t = 0; t < count_; ++t) {
           const Item& it = items_[t];
           const Group& g = groups_[it.group];
@@ -329,7 +351,7 @@ public:
   std::size_t grid_cells() const { return grids_.size(); }
   int count() const { return count_; }
   bool graph_active() const { return graph_ok_[0] || graph_ok_[1]; }
-  bool row_bands() const { return bands_; }
+  bool row_bands() const { return bands_on_; }
   bool fused_fft() const { return fused_; }
   const KernelSetup& kernel() const { return ks_; }
 
@@ -373,7 +395,7 @@ private:
   void enqueue(int type, hipStream_t s) {
     if (type == 1) {
       // every fine-grid cell is written, so the grids need no zero fill
-      if (bands_) {
+      if (bands_on_) {
         if constexpr (DIM == 2) {
           launch_clear_max(cmax_.data(), count_ * kMaxSlots, s);
           launch_band_gather<T>(bjobs_.data(), BandJob<T>{}, count_, total_points_, s);
@@ -399,7 +421,7 @@ private:
         } else {
           g.pfft.set_stream(s);
           if constexpr (DIM == 2) {
-            if (fused_) {
+            if (fused_ && bands_on_) {
               g.pfft.forward_from_h(fs, order_fft(), nullptr, outs, st, s);
               continue;
             }
@@ -498,6 +520,8 @@ private:
   std::size_t max_bins_ = 0;
   bool regular_ = true;  // every group's bins suit the bin-owned reduction
   bool bands_ = true;    // 2D: row-band spreading for every group
+  bool bands_on_ = false;  // ... used for the current points (not too clustered)
+  std::vector<std::vector<uint32_t>> row_host_;  // row offsets per transform (balance check)
   bool fused_ = true;    // ... with the x-FFT inside the band kernel
   IntType tw_len_ = 1;
   DeviceBuffer<Cplx<T>> tw_;

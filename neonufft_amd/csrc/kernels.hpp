@@ -203,6 +203,10 @@ constexpr int kBoundBlocks = 64;
 template <typename T>
 void launch_row_block_count(const PointRec<T, 2>* view, std::size_t m, int n0, int ns,
                             uint32_t* bcount, hipStream_t stream);
+// *out (zeroed by the caller) = the largest sum of bcount over ns consecutive
+// rows (mod n1) of one column block: the most points any cell's bound sees.
+void launch_row_window_max(const uint32_t* bcount, int n1, int ns, uint32_t* out,
+                           hipStream_t stream);
 // Zeroes n strength-maximum slots (a kernel, so graphs may record it). Each
 // job owns kMaxSlots consecutive slots (its maximum is their maximum).
 constexpr int kMaxSlots = 64;

@@ -71,6 +71,34 @@ inline BandLayout plan_band_layout(const IntType* n, int ns, std::size_t acc_byt
   return b;
 }
 
+// Row bands suit a point set when
+//  - work: the fullest band's point range holds at most max(65536, 8 x the
+//    uniform expectation) points (a band is one workgroup with no further
+//    split; subgrid tiles cap a workgroup at 2048 points), and
+//  - precision: no cell's fixed-point bound counts more than kBandMaxWindow
+//    points (window_max); the rounding of the fixed-point sum grows with
+//    that count times the largest strength, so dense grids (many points per
+//    cell) keep the tiles' per-subproblem scale.
+// row_start: n1 + 1 offsets of the row view.
+// An explicit request for bands (check_work false) waives the work rule only.
+constexpr uint32_t kBandMaxWindow = 1024;
+inline bool bands_suitable(const std::vector<uint32_t>& row_start, int n1, int rows, int ns,
+                           uint32_t window_max, bool check_work) {
+  if (window_max > kBandMaxWindow) return false;
+  if (!check_work) return true;
+  const double m = row_start[static_cast<std::size_t>(n1)];
+  const double expect = m * static_cast<double>(rows + ns - 1) / static_cast<double>(n1);
+  const double limit = std::max(65536.0, 8.0 * expect);
+  for (int y0 = 0; y0 < n1; y0 += rows) {
+    const int hi = std::min(y0 + rows, n1);
+    const int lo = y0 - (ns - 1);
+    const double c = lo < 0 ? double(row_start[n1] - row_start[lo + n1]) + row_start[hi]
+                            : double(row_start[hi] - row_start[lo]);
+    if (c > limit) return false;
+  }
+  return true;
+}
+
 inline int log2_exact(IntType v) {
   int l = 0;
   while ((IntType(1) << l) < v) ++l;
@@ -148,17 +176,29 @@ public:
     FoldParams fp{{0, 0, 0}, {1, 1, 1}};
     pts_.build(l, static_cast<std::size_t>(num_nu), fp, geom_, ks_.ns, tune_.max_points, stream_,
                alloc_);
+    bands_on_ = false;
     if constexpr (DIM == 2) {
       if (band_rows_ > 0) {
         rows_.build(pts_, ks_.ns, stream_, alloc_);
         sorted_.ensure(std::max<std::size_t>(pts_.size(), 1), alloc_);
-      } else {
-        tiles_.ensure(std::max<std::size_t>(pts_.nsub(), 1) * tile_cells(geom_), alloc_);
+        row_host_.resize(static_cast<std::size_t>(n_[1]) + 2);
+        check_hip(hipMemcpyAsync(row_host_.data(), rows_.row_start(),
+                                 (row_host_.size() - 1) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, stream_),
+                  "hipMemcpyAsync(row offsets)");
+        check_hip(hipMemcpyAsync(row_host_.data() + n_[1] + 1, rows_.window_max(),
+                                 sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+                  "hipMemcpyAsync(window max)");
       }
-    } else {
-      tiles_.ensure(std::max<std::size_t>(pts_.nsub(), 1) * tile_cells(geom_), alloc_);
     }
     check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    if constexpr (DIM == 2) {
+      bands_on_ = band_rows_ > 0 &&
+                  bands_suitable(row_host_, static_cast<int>(n_[1]), band_rows_, ks_.ns,
+                                 row_host_[static_cast<std::size_t>(n_[1]) + 1],
+                                 tune_.spread_mode == kSpreadAuto);
+    }
+    if (!bands_on_) tiles_.ensure(std::max<std::size_t>(pts_.nsub(), 1) * tile_cells(geom_), alloc_);
   }
 
   // Stage 1 of type 1: form the fine grid from the strengths (overwriting
@@ -174,7 +214,7 @@ public:
   void spread_tiles(const Cplx<T>* in) {
     if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
     if constexpr (DIM == 2) {
-      if (band_rows_ > 0) {
+      if (bands_on_) {
         launch_clear_max(cmax_.data(), kMaxSlots, stream_);
         launch_band_gather<T>(nullptr, band_job(in), 1, pts_.size(), stream_);
         return;
@@ -185,7 +225,7 @@ public:
   }
   void reduce_tiles(bool accumulate) {
     if constexpr (DIM == 2) {
-      if (band_rows_ > 0) {
+      if (bands_on_) {
         const int nb = static_cast<int>((n_[1] + band_rows_ - 1) / band_rows_);
         launch_band_spread<T>(ks_.ns, nullptr, band_job(nullptr), 1, static_cast<std::size_t>(nb),
                               band_lds_bytes(band_rows_, band_.pitch, sizeof(T)), accumulate,
@@ -209,7 +249,7 @@ public:
     if (!out || !out_strides) throw neonufft::InputError("Neonufft:: null output");
     T* fs[3] = {fs_[0].data(), fs_[1].data(), fs_[2].data()};
     if constexpr (DIM == 2) {
-      if (band_.fft) {
+      if (bands_on_ && band_.fft) {
         // gather, then spread + x-FFT per band straight into the pruned
         // FFT's intermediate; y-FFT and deconvolution follow
         if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
@@ -262,8 +302,8 @@ public:
     info->num_points = static_cast<int64_t>(pts_.size());
     info->horner_degree = ks_.horner_deg;
     info->quad_nodes = ks_.quad.nq;
-    info->spread_mode = band_rows_ == 0 ? kSpreadTiles : (band_.fft ? kSpreadBands : kSpreadBandsGrid);
-    info->band_rows = band_rows_;
+    info->spread_mode = !bands_on_ ? kSpreadTiles : (band_.fft ? kSpreadBands : kSpreadBandsGrid);
+    info->band_rows = bands_on_ ? band_rows_ : 0;
   }
 
   Cplx<T>* fine_grid() const { return grid_.data(); }
@@ -308,6 +348,8 @@ private:
   IntType n_[3];
   BinGeom geom_{};
   int band_rows_ = 0;             // 2D row-band spreading (0 = subgrid tiles)
+  bool bands_on_ = false;         // row bands used for the current points
+  std::vector<uint32_t> row_host_;  // row offsets of the view (balance check)
   BandLayout band_;
   DeviceBuffer<Cplx<T>> tw_;      // x-FFT twiddles of the fused type 1
   DeviceBuffer<unsigned long long> cmax_;  // largest strength (row bands)

@@ -131,18 +131,24 @@ public:
     check_hip(hipMemsetAsync(bcount_.data(), 0, n1 * kBoundBlocks * sizeof(uint32_t), stream),
               "hipMemsetAsync(block counts)");
     launch_row_block_count<T>(recs_.data(), m, g.n[0], ns, bcount_.data(), stream);
+    wmax_.ensure(1, alloc);
+    check_hip(hipMemsetAsync(wmax_.data(), 0, sizeof(uint32_t), stream), "hipMemsetAsync(window max)");
+    launch_row_window_max(bcount_.data(), static_cast<int>(n1), ns, wmax_.data(), stream);
   }
 
   const PointRec<T, 2>* recs() const { return recs_.data(); }
   const uint32_t* perm() const { return perm_.data(); }
   const uint32_t* row_start() const { return start_.data(); }
   const uint32_t* block_counts() const { return bcount_.data(); }
+  // device: the largest point count any cell's fixed-point bound sees
+  const uint32_t* window_max() const { return wmax_.data(); }
   std::size_t size() const { return m_; }
 
 private:
   std::size_t m_ = 0;
   DeviceBuffer<PointRec<T, 2>> recs_;
-  DeviceBuffer<uint32_t> perm_, rank_, count_, start_, nchunk_, chunk_start_, scratch_, bcount_;
+  DeviceBuffer<uint32_t> perm_, rank_, count_, start_, nchunk_, chunk_start_, scratch_, bcount_,
+      wmax_;
   DeviceBuffer<Subprob> chunks_;
 };
 

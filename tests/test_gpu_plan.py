@@ -315,3 +315,49 @@ def test_type1_spread_modes(case, order, prec):
     torch.cuda.synchronize()
     got_t = out_t.cpu().numpy().astype(np.complex128).reshape(modes[0], modes[1]).T.ravel()
     assert O.rel_l2(got_t, outs["bands"]) < same
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_type1_clustered_rows_fall_back_to_tiles(prec):
+    """Points packed into a few fine-grid rows (thousands per cell): 2D plans
+    use subgrid tiles for them, also when row bands are asked for (the band's
+    fixed-point bound would count ~1e4 points per cell and lose precision);
+    uniform points set later switch the plan back to row bands, and a batch
+    holding such a transform falls back as a whole. The result stays within
+    the tolerance of the direct sum."""
+    rng = np.random.default_rng(77)
+    m, modes = 200_000, (64, 64)
+    x = rng.uniform(-np.pi, np.pi, m).astype(NPR[prec])
+    y = rng.uniform(0.0, 0.02, m).astype(NPR[prec])  # ~1 of 128 fine rows
+    c = cplx(rng, m, prec)
+    opt = nnf.Options(tol=1e-5 if prec == "f32" else 1e-9)
+    loc = [dev(x, RDT[prec]), dev(y, RDT[prec])]
+    cin = dev(c, CDT[prec])
+    outs = {}
+    for mode in ("auto", "bands"):
+        plan = nnf.Plan(opt, -1, m, loc, modes, spread_mode=mode)
+        assert plan.info()["spread_mode"] == "tiles"
+        out = torch.zeros(int(np.prod(modes)), dtype=CDT[prec], device="cuda")
+        plan.transform_type_1(cin, out)
+        torch.cuda.synchronize()
+        outs[mode] = out.cpu().numpy().astype(np.complex128)
+        xu = [dev(rng.uniform(-np.pi, np.pi, 1000).astype(NPR[prec]), RDT[prec])
+              for _ in range(2)]
+        plan.set_points(1000, xu)
+        assert plan.info()["spread_mode"] == "bands"
+    # same decomposition; points within a bin are ranked by atomics, so the
+    # subproblems (and their fixed-point scales) may differ between plans
+    same = 1e-12 if prec == "f64" else 5e-6
+    assert O.rel_l2(outs["auto"], outs["bands"]) < same
+    idx = sample_modes(modes, rng, 200)
+    ref = O.direct_type1([x.astype(np.float64), y.astype(np.float64)],
+                         c.astype(np.complex128), O.mode_vectors(modes, idx), -1)
+    assert O.rel_l2(outs["auto"][idx], ref) < (4e-5 if prec == "f32" else 1e-8)
+    batch = nnf.BatchPlan(opt, -1, [m, 500], [loc, [loc[0][:500], loc[0][500:1000]]],
+                          [list(modes), [32, 32]], use_graph=False)
+    assert batch.info()["spread_mode"] == "tiles"
+    bo = [torch.zeros(64 * 64, dtype=CDT[prec], device="cuda"),
+          torch.zeros(32 * 32, dtype=CDT[prec], device="cuda")]
+    batch.transform_type_1([cin, cin[:500]], bo)
+    torch.cuda.synchronize()
+    assert O.rel_l2(bo[0].cpu().numpy().astype(np.complex128), outs["auto"]) < same
