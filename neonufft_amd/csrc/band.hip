@@ -1,7 +1,7 @@
 // Row-band spreading for 2D type 1.
 //
-// Same sums as the reference's spread_2d_kernel
-// (src/neonufft/gpu/kernels/spreading_kernel.cu:477-560): cell
+// Same sums as the reference's 2D spreading kernel
+// (src/neonufft/gpu/kernels/spreading_kernel.cu:527-586): cell
 // ((i0x + a) mod n0, (i0y + b) mod n1) receives phi_x(a) phi_y(b) c for every
 // point. The decomposition is MI355X-specific: one workgroup owns a band of
 // whole fine-grid rows, accumulates every point whose kernel rows reach the
