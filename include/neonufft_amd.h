@@ -165,7 +165,9 @@ int nnf_plan_create_tuned(nnf_plan* plan, int precision, int dim, const nnf_opti
  * a plan with these arguments for a list of subgrid tile shapes and
  * per-workgroup point caps, on the given device points, and returns the
  * fastest in `best` (pass it to nnf_plan_create_tuned) with its time per
- * call in `best_ms` (sum over the selected kernels). `reps` timed calls per
+ * call in `best_ms` (sum over the selected kernels). 2D plans that spread
+ * with row bands time their type 1 for a list of band heights instead of
+ * the tile shapes (best->band_rows). `reps` timed calls per
  * candidate. With use_cache non-zero, a result is reused for the same
  * precision, dim, kernel width, fine grid, floor(log2 num_nu) and types in
  * this process. */

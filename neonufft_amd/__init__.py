@@ -158,8 +158,10 @@ def autotune(opt: "Options", sign: int, num_nu: int, loc: Sequence[Pointer],
              use_cache: bool = True) -> dict:
     """Fastest subgrid tile and per-workgroup point cap for a plan with these
     arguments, timed on these device points (extension; no reference
-    counterpart). types: 1 = spreading, 2 = interpolation, 3 = both.
-    Returns {"bin_size": [...], "max_points": int, "ms": float}."""
+    counterpart). types: 1 = spreading, 2 = interpolation, 3 = both. 2D plans
+    spreading with row bands also get the fastest band height (type 1 timed).
+    Returns {"bin_size": [...], "max_points": int, "band_rows": int, "ms": float}
+    (band_rows 0 = default)."""
     L = _L.lib()
     dim = len(modes)
     prec = _dtype_code(dtype) if dtype is not None else _precision_of(loc[0])
@@ -170,7 +172,7 @@ def autotune(opt: "Options", sign: int, num_nu: int, loc: Sequence[Pointer],
                           _i64p(modes), ctypes.c_void_p(_stream(stream)), int(types), int(reps),
                           int(bool(use_cache)), ctypes.byref(best), ctypes.byref(ms)))
     return {"bin_size": list(best.bin_size)[:dim], "max_points": int(best.max_points),
-            "ms": float(ms.value)}
+            "band_rows": int(best.band_rows), "ms": float(ms.value)}
 
 
 class Plan:
@@ -195,6 +197,7 @@ class Plan:
                 raise InputError(f"bin_size must be a sequence or 'auto', not {bin_size!r}")
             t = autotune(opt, sign, num_nu, loc, modes, stream=stream, dtype=dtype)
             bin_size, max_points = t["bin_size"], t["max_points"]
+            band_rows = band_rows or t["band_rows"]
         self.dim = len(modes)
         self.modes = [int(m) for m in modes]
         self.precision = _dtype_code(dtype) if dtype is not None else _precision_of(loc[0])

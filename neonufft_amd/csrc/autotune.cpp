@@ -5,7 +5,9 @@
 // For a plan's precision, dimension, kernel width, fine grid and points it
 // times the spreading and/or interpolation kernels for a list of subgrid tile
 // shapes and per-workgroup point caps, on the caller's own points, and keeps
-// the fastest. Results are cached in-process per (precision, dim, ns, fine
+// the fastest. 2D plans that spread with row bands time their type 1 for a
+// list of band heights instead (the tile shape then only serves
+// interpolation), and the two choices are returned together. Results are cached in-process per (precision, dim, ns, fine
 // grid, log2 of the point count, kernel selection): the point distribution is
 // not part of the key, so a cached answer assumes similar clustering.
 #include <hip/hip_runtime.h>
@@ -94,6 +96,7 @@ std::pair<nnf_tuning, double> tune(const neonufft::Options& opt, int sign, IntTy
   nnf_tuning best{};
   double best_ms = -1.0;
   std::vector<std::array<int64_t, 4>> seen;
+  bool bands = false;  // type 1 spreads with row bands: tuned separately below
   for (const Candidate& c : candidates(DIM)) {
     PlanTuning t;
     for (int d = 0; d < 3; ++d) t.bin[d] = d < DIM ? c.bin[d] : 0;
@@ -108,8 +111,9 @@ std::pair<nnf_tuning, double> tune(const neonufft::Options& opt, int sign, IntTy
     for (const auto& s : seen) dup = dup || s == key;
     if (dup) continue;
     seen.push_back(key);
+    bands = info.spread_mode != kSpreadTiles;
     double ms = 0.0;
-    if (types & 1) {
+    if ((types & 1) && !bands) {
       plan.spread(strengths.data(), false);
       ms += ev.time_ms(stream, reps, [&] { plan.spread(strengths.data(), true); });
     }
@@ -120,6 +124,49 @@ std::pair<nnf_tuning, double> tune(const neonufft::Options& opt, int sign, IntTy
       for (int d = 0; d < 3; ++d)
         best.bin_size[d] = d < DIM ? static_cast<int32_t>(info.bin_size[d]) : 0;
       best.max_points = static_cast<int32_t>(c.max_points);
+    }
+    if (!(types & 2) && bands) break;  // nothing a tile shape changes is timed
+  }
+  if constexpr (DIM == 2) {
+    if ((types & 1) && bands) {
+      // band heights: the default, then powers of two up to the LDS budget
+      // (heights the plan clamps onto an already timed one are skipped)
+      std::size_t nm = 1;
+      for (int d = 0; d < DIM; ++d) nm *= static_cast<std::size_t>(modes[d]);
+      DeviceBuffer<Cplx<T>> out(nm, alloc);
+      const IntType st[3] = {1, modes[0], 1};
+      double band_ms = -1.0;
+      int band_rows = 0;
+      std::vector<int> timed;
+      for (int rows : {0, 1, 2, 4, 8, 16, 32}) {
+        PlanTuning t;
+        t.spread_mode = kSpreadBands;
+        t.band_rows = rows;
+        try {
+          PlanImpl<T, DIM> plan(opt, sign, num_nu, loc, modes, stream, alloc, t);
+          nnf_plan_info info;
+          plan.info(&info);
+          if (info.spread_mode == kSpreadTiles) break;  // the points do not suit bands
+          bool dup = false;
+          for (int r : timed) dup = dup || r == info.band_rows;
+          if (dup) continue;
+          timed.push_back(info.band_rows);
+          const double ms = ev.time_ms(stream, reps, [&] {
+            plan.transform_type_1(strengths.data(), out.data(), st);
+          });
+          if (band_ms < 0.0 || ms < band_ms) {
+            band_ms = ms;
+            band_rows = info.band_rows;
+          }
+        } catch (const neonufft::InputError&) {
+          break;
+        }
+      }
+      if (band_ms >= 0.0) {
+        best.spread_mode = NNF_SPREAD_AUTO;
+        best.band_rows = band_rows;
+        best_ms = (best_ms < 0.0 ? 0.0 : best_ms) + band_ms;
+      }
     }
   }
   return {best, best_ms};

@@ -30,6 +30,8 @@ def test_autotune_result_and_plan(dim, modes, m):
     t = nnf.autotune(opt, -1, m, x, modes, types=3, reps=2)
     assert t["ms"] > 0.0
     assert t["max_points"] in (1024, 2048)
+    # 2D plans spread with row bands: their height is tuned too
+    assert (t["band_rows"] > 0) == (dim == 2)
     assert len(t["bin_size"]) == dim and all(b > 0 for b in t["bin_size"])
     # cached: same answer, same time
     t2 = nnf.autotune(opt, -1, m, x, modes, types=3, reps=2)
@@ -38,6 +40,8 @@ def test_autotune_result_and_plan(dim, modes, m):
     ref = nnf.Plan(opt, -1, m, x, modes)
     tuned = nnf.Plan(opt, -1, m, x, modes, bin_size="auto")
     assert tuned.info()["bin_size"] == t["bin_size"]
+    if dim == 2:
+        assert tuned.info()["band_rows"] == t["band_rows"]
     nm = int(np.prod(modes))
     f0 = torch.empty(nm, dtype=torch.complex64, device="cuda")
     f1 = torch.empty_like(f0)
