@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(kBandThreads<T>)
 #pragma unroll
   for (int w = 0; w < NW; ++w) bc = max(bc, part[w][threadIdx.x & 63]);
   const T bound = static_cast<T>(wave_max(bc)) * wave_max(cm);
-  const T one = static_cast<T>(1ull << FixedAcc<T>::bits);
+  const T one = FixedAcc<T>::one;
   const T scale = bound > T(0) ? one / bound : T(0);
 
   for (uint32_t q0 = 0; q0 < total; q0 += NT * KB) {

@@ -65,25 +65,27 @@ __device__ inline int unpack_cell(uint32_t cell, int d) {
 // The tile accumulates in fixed point with integer LDS atomics: on gfx950,
 // ds_add_u32 sustains 5-9e12 lane-ops/s while ds_add_f32 runs at ~2e11.
 // Each subproblem picks its own scale from a rigorous bound on any tile value,
-// bound = sum over its points of |Re c| + |Im c| (kernel weights are <= 1), so
-// fp32 plans keep 30 fractional-range bits in int32 and fp64 plans 62 bits in
-// int64. Finished tiles are converted back once per cell.
+// bound = sum over its points of |Re c| + |Im c| (kernel weights are <= 1),
+// mapped to FixedAcc<T>::one: the full int32 (fp32 plans) / int64 (fp64
+// plans) range less a margin for the rounding of each added term (at most
+// one half per add; a cell receives at most a few thousand adds). Finished
+// tiles are converted back once per cell.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct FixedAcc;
 template <>
 struct FixedAcc<float> {
   using type = int;
-  static constexpr int bits = 30;
+  static constexpr float one = 2147483648.0f - 2097152.0f;  // 2^31 - 2^21
 };
 template <>
 struct FixedAcc<double> {
   using type = long long;
-  static constexpr int bits = 62;
+  static constexpr double one = 9223372036854775808.0 - 8796093022208.0;  // 2^63 - 2^43
 };
 
 // Round to the nearest integer (floor(v + 0.5)) in one VALU instruction; the
-// compiler emits rndne + cvt (two) for __float2int_rn. |v| <= 2^30 here.
+// compiler emits rndne + cvt (two) for __float2int_rn. |v| < 2^31 here.
 __device__ inline int to_fixed(float v) {
   int r;
   asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));

@@ -114,7 +114,7 @@ __device__ __forceinline__ void spread_subproblem(const SpreadJob<T, DIM>& job,
     }
   }
   const T total = block_sum(bound, wsum);  // also orders the tile zero fill
-  const T one = static_cast<T>(1ull << FixedAcc<T>::bits);
+  const T one = FixedAcc<T>::one;
   const T scale = total > T(0) ? one / total : T(0);
 
 #pragma unroll
