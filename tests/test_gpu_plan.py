@@ -268,6 +268,8 @@ BAND_T1 = [
     ((16, 128), 2000, 1e-6, 1, 0),
     ((256, 16), 4000, 1e-5, 1, 0),
     ((64, 64), 3000, 1e-10, 1, 0),
+    # fp64 rows of 4096 cells: one row per band (LDS budget)
+    ((2048, 8), 2000, 1e-9, 1, 0),
 ]
 
 
