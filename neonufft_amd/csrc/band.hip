@@ -778,8 +778,6 @@ void launch_band_spread(int ns, const BandJob<T>* jobs, const BandJob<T>& single
   if (total_bands == 0) return;
   if (fft && accumulate) throw neonufft::InternalError("fused band FFT cannot accumulate");
   if (total_bands > 0x7fffffffu) throw neonufft::InputError("Neonufft:: too many row bands");
-  constexpr int DIM = 2;
-  (void)DIM;
 #define NNF_BAND_CASE(NSV)                                                                    \
   case NSV:                                                                                  \
     band_spread_ns<T, NSV>(jobs, single, njobs, total_bands, lds_bytes, accumulate, fft, es,   \
