@@ -149,19 +149,31 @@ def test_batch_graph_replays(prec, shapes):
         assert not eb.info()["graph"]
 
 
-def test_batch_set_points():
-    """New points (and point counts) through set_points match a fresh plan."""
-    prec, dim = "f32", 3
-    opt = nnf.Options(tol=1e-5)
+# new point counts per transform for test_batch_set_points (0 empties one,
+# larger counts regrow the point views)
+NUM2 = {3: [120, 0, 3000, 45], "2f": [0, 5000, 700, 40, 2500]}
+
+
+@pytest.mark.parametrize("dim,prec,use_graph",
+                         [(3, "f32", False), ("2f", "f32", False), ("2f", "f32", True),
+                          ("2f", "f64", True)])
+def test_batch_set_points(dim, prec, use_graph):
+    """New points (and point counts) through set_points match a fresh plan,
+    also after a graph was captured for the old points."""
+    opt = nnf.Options(tol=1e-5 if prec == "f32" else 1e-9)
     spec, locs, cs, fs = _setup(dim, prec, 3)
+    pdim = 2 if dim == "2f" else dim
     modes = [list(md) for md, _ in spec]
-    batch = nnf.BatchPlan(opt, 1, [m for _, m in spec], locs, modes, dtype=RDT[prec])
+    batch = nnf.BatchPlan(opt, 1, [m for _, m in spec], locs, modes, dtype=RDT[prec],
+                          use_graph=use_graph)
+    outs = [torch.empty(int(np.prod(md)), dtype=CDT[prec], device="cuda") for md, _ in spec]
+    batch.transform_type_1(cs, outs)
+    torch.cuda.synchronize()
     rng = np.random.default_rng(8)
-    num2 = [120, 0, 3000, 45]
-    locs2 = [[dev(v, RDT[prec]) for v in points(rng, dim, m, prec)] for m in num2]
+    num2 = NUM2[dim]
+    locs2 = [[dev(v, RDT[prec]) for v in points(rng, pdim, m, prec)] for m in num2]
     cs2 = [dev(cplx(rng, m, prec), CDT[prec]) for m in num2]
     batch.set_points(num2, locs2)
-    outs = [torch.empty(int(np.prod(md)), dtype=CDT[prec], device="cuda") for md, _ in spec]
     batch.transform_type_1(cs2, outs)
     for t, (md, _) in enumerate(spec):
         plan = nnf.Plan(opt, 1, num2[t], locs2[t], md, dtype=RDT[prec])
