@@ -343,10 +343,19 @@ def test_type1_clustered_rows_fall_back_to_tiles(prec):
         plan.transform_type_1(cin, out)
         torch.cuda.synchronize()
         outs[mode] = out.cpu().numpy().astype(np.complex128)
-        xu = [dev(rng.uniform(-np.pi, np.pi, 1000).astype(NPR[prec]), RDT[prec])
-              for _ in range(2)]
+        xu_h = [rng.uniform(-np.pi, np.pi, 1000).astype(NPR[prec]) for _ in range(2)]
+        xu = [dev(v, RDT[prec]) for v in xu_h]
         plan.set_points(1000, xu)
         assert plan.info()["spread_mode"] == "bands"
+        # the plan switched from tiles to bands: its result follows the new points
+        cu = cplx(rng, 1000, prec)
+        ou = torch.zeros(int(np.prod(modes)), dtype=CDT[prec], device="cuda")
+        plan.transform_type_1(dev(cu, CDT[prec]), ou)
+        torch.cuda.synchronize()
+        ref_u = O.direct_type1([v.astype(np.float64) for v in xu_h], cu.astype(np.complex128),
+                               O.mode_vectors(modes, np.arange(int(np.prod(modes)))), -1)
+        assert O.rel_l2(ou.cpu().numpy().astype(np.complex128), ref_u) < (
+            4e-5 if prec == "f32" else 1e-8)
     # same decomposition; points within a bin are ranked by atomics, so the
     # subproblems (and their fixed-point scales) may differ between plans
     same = 1e-12 if prec == "f64" else 5e-6
