@@ -356,6 +356,13 @@ def test_type1_clustered_rows_fall_back_to_tiles(prec):
                                O.mode_vectors(modes, np.arange(int(np.prod(modes)))), -1)
         assert O.rel_l2(ou.cpu().numpy().astype(np.complex128), ref_u) < (
             4e-5 if prec == "f32" else 1e-8)
+        # and back to the clustered points: tiles again, same result as before
+        plan.set_points(m, loc)
+        assert plan.info()["spread_mode"] == "tiles"
+        plan.transform_type_1(cin, ou)
+        torch.cuda.synchronize()
+        again = ou.cpu().numpy().astype(np.complex128)
+        assert O.rel_l2(again, outs[mode]) < (1e-12 if prec == "f64" else 5e-6)
     # same decomposition; points within a bin are ranked by atomics, so the
     # subproblems (and their fixed-point scales) may differ between plans
     same = 1e-12 if prec == "f64" else 5e-6
