@@ -125,32 +125,21 @@ __global__ void __launch_bounds__(kRowBlock)
   atomicMax(out, s);
 }
 
-__global__ void clear_kernel(unsigned long long* __restrict__ p, int n) {
-  const int i = threadIdx.x + blockIdx.x * blockDim.x;
-  if (i < n) p[i] = 0ull;
+__global__ void clear_kernel(unsigned long long* __restrict__ p, std::size_t n) {
+  for (std::size_t i = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * blockDim.x; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+    p[i] = 0ull;
+  }
 }
 
-// Largest |Re c| + |Im c| of a job, as the bits of a non-negative T (whose
-// unsigned order is the numeric order).
-__device__ inline void max_into(void* slot, float v) {
-  atomicMax(static_cast<unsigned int*>(slot), __float_as_uint(v));
+// Running maximum of a non-negative T through the unsigned order of its bits
+// (the same as the numeric order for non-negative values).
+__device__ inline void max_into(float* slot, float v) {
+  atomicMax(reinterpret_cast<unsigned int*>(slot), __float_as_uint(v));
 }
-__device__ inline void max_into(void* slot, double v) {
-  atomicMax(static_cast<unsigned long long*>(slot),
+__device__ inline void max_into(double* slot, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(slot),
             static_cast<unsigned long long>(__double_as_longlong(v)));
-}
-// Slot k of a job's kMaxSlots maxima (each 8 bytes).
-__device__ inline void* max_slot(void* base, int k, float) {
-  return static_cast<unsigned long long*>(base) + k;
-}
-__device__ inline void* max_slot(void* base, int k, double) {
-  return static_cast<unsigned long long*>(base) + k;
-}
-__device__ inline float max_from(const void* slot, float) {
-  return __uint_as_float(*static_cast<const unsigned int*>(slot));
-}
-__device__ inline double max_from(const void* slot, double) {
-  return __longlong_as_double(static_cast<long long>(*static_cast<const unsigned long long*>(slot)));
 }
 
 // Index of the job owning flattened item `i` (jobs sorted by `off`).
@@ -169,65 +158,97 @@ __device__ inline int find_job(const BandJob<T>* jobs, int njobs, std::size_t i,
   return lo;
 }
 
-// sorted[p] = in[perm[p]] over the points of every job (flattened), and the
-// largest |Re c| + |Im c| of every job into its slot (cleared beforehand):
-// one atomic per wave when all of the wave's items belong to one job (the
-// common case), else one per item.
+// sorted[p] = in[perm[p]] over the points of every job (flattened), and per
+// first kernel row the sum and the largest |Re c| + |Im c| of its points into
+// the job's (cleared) rowsum / rowmax. Consecutive lanes take consecutive view
+// points (sorted by row), so one row's points are a run of lanes: a segmented
+// wave scan combines each run and the run's last lane issues one atomic add
+// and one atomic max.
 template <typename T>
 __global__ void __launch_bounds__(kRowBlock)
     band_gather_kernel(const BandJob<T>* __restrict__ jobs, BandJob<T> single, int njobs,
                        std::size_t total) {
   const std::size_t base = static_cast<std::size_t>(blockIdx.x) * (kRowBlock * kGatherItems);
   if (base >= total) return;
-  const int j0 = jobs ? find_job(jobs, njobs, base, false) : 0;
-  int j = j0;
+  int j = jobs ? find_job(jobs, njobs, base, false) : 0;
   BandJob<T> job = jobs ? jobs[j] : single;
   // all index loads, then all strength loads, then the stores
   Cplx<T>* dst[kGatherItems];
   const Cplx<T>* src[kGatherItems];
   int jk[kGatherItems];
+  uint32_t pp[kGatherItems];
 #pragma unroll
   for (int k = 0; k < kGatherItems; ++k) {
     const std::size_t i = base + threadIdx.x + static_cast<std::size_t>(k) * kRowBlock;
     dst[k] = nullptr;
     src[k] = nullptr;
-    jk[k] = j0;
+    jk[k] = j;
+    pp[k] = 0u;
     if (i < total) {
       while (i >= job.pt_off + job.m) job = jobs[++j];
       const std::size_t p = i - job.pt_off;
       dst[k] = job.sorted + p;
       src[k] = job.in + job.perm[p];
       jk[k] = j;
+      pp[k] = static_cast<uint32_t>(p);
     }
   }
   Cplx<T> v[kGatherItems];
 #pragma unroll
   for (int k = 0; k < kGatherItems; ++k) v[k] = src[k] ? *src[k] : Cplx<T>{T(0), T(0)};
-  T mx = T(0);
-  bool mixed = false;
 #pragma unroll
   for (int k = 0; k < kGatherItems; ++k) {
-    if (dst[k]) {
-      *dst[k] = v[k];
-      mx = max(mx, fabs(v[k].x) + fabs(v[k].y));
-      mixed = mixed || jk[k] != j0;
-    }
+    if (dst[k]) *dst[k] = v[k];
   }
-  // a job's maximum is spread over kMaxSlots slots (slot = workgroup mod
-  // kMaxSlots) so that its many workgroups do not queue on one address; no
-  // barrier here, which would also wait for the stores above
-  const int slot = static_cast<int>(blockIdx.x % kMaxSlots);
-  if (__ballot(mixed) == 0) {
+  // First kernel row of every item: the last row whose view offset is <= p
+  // (binary search over the job's n1 + 1 row offsets, all items in step).
+  int lo[kGatherItems], hi[kGatherItems];
+  const uint32_t* rs[kGatherItems];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
-    if ((threadIdx.x & 63) == 0) max_into(max_slot(jobs ? jobs[j0].cmax : single.cmax, slot, mx), mx);
-  } else {
+  for (int k = 0; k < kGatherItems; ++k) {
+    const BandJob<T>& jb = jobs ? jobs[jk[k]] : single;
+    rs[k] = jb.row_start;
+    lo[k] = 0;
+    hi[k] = dst[k] ? jb.n1 : 1;
+  }
+  for (bool more = true; more;) {
+    more = false;
 #pragma unroll
     for (int k = 0; k < kGatherItems; ++k) {
-      if (dst[k]) {
-        const T a = fabs(v[k].x) + fabs(v[k].y);
-        max_into(max_slot(jobs[jk[k]].cmax, slot, a), a);
+      if (hi[k] - lo[k] > 1) {
+        const int mid = (lo[k] + hi[k]) >> 1;
+        if (rs[k][mid] <= pp[k]) {
+          lo[k] = mid;
+        } else {
+          hi[k] = mid;
+        }
+        more = more || hi[k] - lo[k] > 1;
       }
+    }
+  }
+  const int lane = static_cast<int>(threadIdx.x & 63);
+#pragma unroll
+  for (int k = 0; k < kGatherItems; ++k) {
+    // (job, row) key: rows and jobs are both below 2^16
+    const uint32_t key = dst[k] ? (static_cast<uint32_t>(jk[k]) << 16) | static_cast<uint32_t>(lo[k])
+                                : 0xffffffffu;
+    T s = dst[k] ? fabs(v[k].x) + fabs(v[k].y) : T(0);
+    T mx = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const T ts = __shfl_up(s, off, 64);
+      const T tm = __shfl_up(mx, off, 64);
+      const uint32_t kk = __shfl_up(key, off, 64);
+      if (lane >= off && kk == key) {
+        s += ts;
+        mx = max(mx, tm);
+      }
+    }
+    const uint32_t next = __shfl_down(key, 1, 64);
+    if (dst[k] && (lane == 63 || next != key)) {
+      const BandJob<T>& jb = jobs ? jobs[jk[k]] : single;
+      atomicAdd(jb.rowsum + lo[k], s);
+      max_into(jb.rowmax + lo[k], mx);
     }
   }
 }
@@ -557,35 +578,41 @@ __global__ void __launch_bounds__(kBandThreads<T>)
   const Cplx<T>* __restrict__ str = job.sorted;
 
   // Fixed-point bound: cell (r, x) only receives points whose first kernel
-  // row lies in [r - NS + 1, r] and whose kernel x-window meets x's column
-  // block, each at most the job's largest |Re c| + |Im c|. Lane k of wave w
-  // takes the band rows w, w + NW, ... and the maximum over them of block k's
-  // counts summed over that NS-row window; after the barrier every wave takes
-  // the maxima over waves and blocks (and of the job's maximum slots).
+  // row lies in [r - NS + 1, r] (its NS-row window), each weighted by at most
+  // one. Two bounds hold for it: (points of the window whose kernel x-window
+  // meets x's column block) x (the largest |Re c| + |Im c| in the window's
+  // rows), and the sum of |Re c| + |Im c| over the window's points (row
+  // maxima and sums from the gather). The smaller is taken per (row, block),
+  // so a few strengths far above the rest coarsen only the scale of the bands
+  // near them, and by at most the window sum. Lane k of wave w takes the band
+  // rows w, w + NW, ... and column block k; after the barrier every wave
+  // takes the maximum over waves and blocks.
   constexpr int NW = NT / 64;
-  __shared__ uint32_t part[NW][kBoundBlocks];
+  __shared__ T part[NW][kBoundBlocks];
   {
     const int k = static_cast<int>(threadIdx.x & 63);
     const int w = static_cast<int>(threadIdx.x >> 6);
-    uint32_t bm = 0;
+    T bm = T(0);
     for (int r = w; r < rk; r += NW) {
       uint32_t bc = 0;
+      T ws = T(0), wm = T(0);
 #pragma unroll
       for (int j = 0; j < NS; ++j) {
         int row = y0 + r - j;
         row = row < 0 ? row + n1 : row;
         bc += job.bcount[static_cast<std::size_t>(row) * kBoundBlocks + k];
+        ws += job.rowsum[row];
+        wm = max(wm, job.rowmax[row]);
       }
-      bm = max(bm, bc);
+      bm = max(bm, min(static_cast<T>(bc) * wm, ws));
     }
     part[w][k] = bm;
   }
-  const T cm = max_from(static_cast<const unsigned long long*>(job.cmax) + (threadIdx.x & 63), T(0));
   __syncthreads();  // zeroed LDS and the partial maxima
-  uint32_t bc = 0;
+  T bpart = T(0);
 #pragma unroll
-  for (int w = 0; w < NW; ++w) bc = max(bc, part[w][threadIdx.x & 63]);
-  const T bound = static_cast<T>(wave_max(bc)) * wave_max(cm);
+  for (int w = 0; w < NW; ++w) bpart = max(bpart, part[w][threadIdx.x & 63]);
+  const T bound = wave_max(bpart);
   const T one = FixedAcc<T>::one;
   const T scale = bound > T(0) ? one / bound : T(0);
 
@@ -753,9 +780,9 @@ void launch_row_window_max(const uint32_t* bcount, int n1, int ns, uint32_t* out
   check_launch("row_window_max_kernel");
 }
 
-void launch_clear_max(unsigned long long* slots, int n, hipStream_t stream) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(clear_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, slots, n);
+void launch_clear_bounds(unsigned long long* words, std::size_t n, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(clear_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, stream, words, n);
   check_launch("clear_kernel");
 }
 

@@ -87,6 +87,9 @@ public:
         use_graph_(use_graph) {
     if (!alloc_) throw neonufft::InputError("Neonufft:: null allocator");
     if (count <= 0) throw neonufft::InputError("Neonufft:: batch count must be positive");
+    // one launch dimension (gridDim.y / z <= 65535) indexes the transforms of
+    // the batched reductions and up / downsampling
+    if (count > 65535) throw neonufft::InputError("Neonufft:: at most 65535 transforms per batch");
     if (!num_nu || !modes) throw neonufft::InputError("Neonufft:: null argument");
     ks_ = make_kernel_setup(opt_.tol, opt_.upsampfac, std::is_same_v<T, float>);
     count_ = static_cast<int>(count);
@@ -183,7 +186,13 @@ public:
     host_io_.resize(count_);
     up_sjobs_.init(sizeof(SpreadJob<T, DIM>) * count_);
     if (bands_) {
-      cmax_.reset(static_cast<std::size_t>(count_) * kMaxSlots, alloc_);
+      // per transform: n1 row sums and n1 row maxima of |c|
+      std::size_t words = 0;
+      for (int t = 0; t < count_; ++t) {
+        items_[t].bound_off = words;
+        words += band_bound_words(static_cast<int>(items_[t].n[1]), sizeof(T));
+      }
+      bounds_.reset(words, alloc_);
       bjobs_.reset(count_, alloc_);
       host_bjobs_.resize(count_);
       rows_.resize(count_);
@@ -292,7 +301,8 @@ public:
           j.perm = rows_[t].perm();
           j.row_start = rows_[t].row_start();
           j.bcount = rows_[t].block_counts();
-          j.cmax = cmax_.data() + static_cast<std::size_t>(t) * kMaxSlots;
+          j.rowsum = reinterpret_cast<T*>(bounds_.data() + it.bound_off);
+          j.rowmax = j.rowsum + g.n[1];
           j.in = in[t];
           j.sorted = sorted_.data() + it.pt_off;
           j.grid = grids_.data() + it.grid_off;
@@ -364,6 +374,7 @@ private:
     std::size_t tiles_off = 0;  // first tile cell of this transform in tiles_
     std::size_t pt_off = 0;     // first point of this transform over all transforms
     std::size_t band_off = 0;   // first row band of this transform (row bands)
+    std::size_t bound_off = 0;  // first word of this transform's row sums / maxima (row bands)
     int group = 0;
     int member = 0;             // index of this transform inside its group
   };
@@ -397,7 +408,7 @@ private:
       // every fine-grid cell is written, so the grids need no zero fill
       if (bands_on_) {
         if constexpr (DIM == 2) {
-          launch_clear_max(cmax_.data(), count_ * kMaxSlots, s);
+          launch_clear_bounds(bounds_.data(), bounds_.size(), s);
           launch_band_gather<T>(bjobs_.data(), BandJob<T>{}, count_, total_points_, s);
           launch_band_spread<T>(ks_.ns, bjobs_.data(), BandJob<T>{}, count_, total_bands_,
                                 band_lds_, false, fused_, ks_.es, s);
@@ -525,7 +536,7 @@ private:
   bool fused_ = true;    // ... with the x-FFT inside the band kernel
   IntType tw_len_ = 1;
   DeviceBuffer<Cplx<T>> tw_;
-  DeviceBuffer<unsigned long long> cmax_;  // per transform: largest strength
+  DeviceBuffer<unsigned long long> bounds_;  // per transform: row sums + row maxima of |c|
   std::size_t total_bands_ = 0, band_lds_ = 0;
   std::vector<RowView<T>> rows_;  // row-sorted views (row bands)
   DeviceBuffer<Cplx<T>> sorted_;  // strengths of all transforms in view order

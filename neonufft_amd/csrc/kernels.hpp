@@ -182,7 +182,8 @@ struct BandJob {
   const uint32_t* perm;        // row-sorted position -> caller's point index
   const uint32_t* row_start;   // n1 + 1 offsets into the view
   const uint32_t* bcount;      // n1 x kBoundBlocks: points per (first row, column block)
-  void* cmax;                  // kMaxSlots x 8 bytes: largest |Re c| + |Im c| (bits of T)
+  T* rowsum;                   // n1: sum of |Re c| + |Im c| over the points of each first row
+  T* rowmax;                   // n1: largest |Re c| + |Im c| among them
   const Cplx<T>* in;           // caller's strengths
   Cplx<T>* sorted;             // strengths in view order (gather output)
   Cplx<T>* grid;               // fine grid, x fastest
@@ -207,14 +208,18 @@ void launch_row_block_count(const PointRec<T, 2>* view, std::size_t m, int n0, i
 // rows (mod n1) of one column block: the most points any cell's bound sees.
 void launch_row_window_max(const uint32_t* bcount, int n1, int ns, uint32_t* out,
                            hipStream_t stream);
-// Zeroes n strength-maximum slots (a kernel, so graphs may record it). Each
-// job owns kMaxSlots consecutive slots (its maximum is their maximum).
-constexpr int kMaxSlots = 64;
-void launch_clear_max(unsigned long long* slots, int n, hipStream_t stream);
+// Zeroes n 8-byte words of strength bounds (a kernel, so graphs may record
+// it). Each job owns band_bound_words(n1, sizeof(T)) words: its n1 row sums,
+// then its n1 row maxima.
+void launch_clear_bounds(unsigned long long* words, std::size_t n, hipStream_t stream);
+inline std::size_t band_bound_words(int n1, std::size_t tbytes) {
+  return (2 * static_cast<std::size_t>(n1) * tbytes + 7) / 8;
+}
 
-// Strengths into view order (sorted[p] = in[perm[p]]) for every job, and
-// each job's largest |Re c| + |Im c| into its (cleared) cmax slot; `jobs` is
-// a device table of njobs entries sorted by pt_off, or null to use `single`.
+// Strengths into view order (sorted[p] = in[perm[p]]) for every job, and per
+// first kernel row the sum and the largest of |Re c| + |Im c| over the row's
+// points into the job's (cleared) rowsum / rowmax; `jobs` is a device table
+// of njobs entries sorted by pt_off, or null to use `single`.
 // total_points = sum of m.
 template <typename T>
 void launch_band_gather(const BandJob<T>* jobs, const BandJob<T>& single, int njobs,

@@ -163,7 +163,7 @@ public:
     if constexpr (DIM == 2) {
       band_ = plan_band_layout(n_, ks_.ns, sizeof(T), tune_, 256);
       band_rows_ = band_.rows;
-      if (band_rows_ > 0) cmax_.reset(kMaxSlots, alloc_);
+      if (band_rows_ > 0) bounds_.reset(band_bound_words(static_cast<int>(n_[1]), sizeof(T)), alloc_);
       if (band_.fft) upload_twiddles<T>(tw_, n_[0], sign_, stream_, alloc_);
     }
     set_points(num_nu, loc);
@@ -215,7 +215,7 @@ public:
     if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
     if constexpr (DIM == 2) {
       if (bands_on_) {
-        launch_clear_max(cmax_.data(), kMaxSlots, stream_);
+        launch_clear_bounds(bounds_.data(), bounds_.size(), stream_);
         launch_band_gather<T>(nullptr, band_job(in), 1, pts_.size(), stream_);
         return;
       }
@@ -253,7 +253,7 @@ public:
         // gather, then spread + x-FFT per band straight into the pruned
         // FFT's intermediate; y-FFT and deconvolution follow
         if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
-        launch_clear_max(cmax_.data(), kMaxSlots, stream_);
+        launch_clear_bounds(bounds_.data(), bounds_.size(), stream_);
         launch_band_gather<T>(nullptr, band_job(in), 1, pts_.size(), stream_);
         const int nb = static_cast<int>((n_[1] + band_rows_ - 1) / band_rows_);
         launch_band_spread<T>(ks_.ns, nullptr, band_job(nullptr), 1, static_cast<std::size_t>(nb),
@@ -318,7 +318,8 @@ private:
     j.perm = rows_.perm();
     j.row_start = rows_.row_start();
     j.bcount = rows_.block_counts();
-    j.cmax = cmax_.data();
+    j.rowsum = reinterpret_cast<T*>(bounds_.data());
+    j.rowmax = j.rowsum + n_[1];
     j.in = in;
     j.sorted = sorted_.data();
     j.grid = grid_.data();
@@ -352,7 +353,7 @@ private:
   std::vector<uint32_t> row_host_;  // row offsets of the view (balance check)
   BandLayout band_;
   DeviceBuffer<Cplx<T>> tw_;      // x-FFT twiddles of the fused type 1
-  DeviceBuffer<unsigned long long> cmax_;  // largest strength (row bands)
+  DeviceBuffer<unsigned long long> bounds_;  // row sums + row maxima of |c| (row bands)
   RowView<T> rows_;               // row-sorted view (row bands)
   DeviceBuffer<Cplx<T>> sorted_;  // strengths in view order (row bands)
   DeviceBuffer<Cplx<T>> grid_;

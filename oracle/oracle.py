@@ -352,16 +352,20 @@ def postphase(s: Sequence[np.ndarray], xbar, sbar, scale, sign: int, ns: int, be
 
 def nufft3(x: Sequence[np.ndarray], c: np.ndarray, s: Sequence[np.ndarray], sign: int,
            tol: float, upsampfac: float = 2.0, recenter_threshold: float = 0.1,
-           nq: Optional[int] = None):
+           nq: Optional[int] = None, bounds=None):
     """Type 3 via prephase x spread x upsample/FFT/interp x postphase
-    (gpu/plan_t3_impl.hpp:74-252)."""
+    (gpu/plan_t3_impl.hpp:74-252). bounds = (input_min, input_max,
+    output_min, output_max) sizes the grids as the bounds constructor does
+    (gpu/plan_t3_impl.hpp:121-138); None takes the points' own extents (the
+    points constructor, gpu/plan_t3_impl.hpp:74-119)."""
     dim = len(x)
     ns, beta = kernel_params(tol, upsampfac)
     xs = [np.asarray(x[d], dtype=np.float64) for d in range(dim)]
     ss = [np.asarray(s[d], dtype=np.float64) for d in range(dim)]
-    gi = GridInfoT3.make(ns, upsampfac, recenter_threshold,
-                         [v.min() for v in xs], [v.max() for v in xs],
-                         [v.min() for v in ss], [v.max() for v in ss])
+    if bounds is None:
+        bounds = ([v.min() for v in xs], [v.max() for v in xs],
+                  [v.min() for v in ss], [v.max() for v in ss])
+    gi = GridInfoT3.make(ns, upsampfac, recenter_threshold, *[list(b) for b in bounds])
     h = []
     gam = []
     for d in range(dim):

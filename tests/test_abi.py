@@ -114,3 +114,14 @@ def test_cpp_dropin_example_compiles_and_links(tmp_path):
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True)  # no arguments: no GPU calls
     assert r.returncode == 0 and "usage" in r.stdout
+
+
+def test_cpp_dropin_hooks_load():
+    """The drop-in caller built as a shared object (tests/cpp/Makefile, run by
+    build()) loads on the CPU and exports the hooks the GPU tests call."""
+    path = os.path.join(ROOT, "tests", "cpp", "libdropin.so")
+    assert os.path.exists(path), "run build()"
+    L.lib()
+    lib = ctypes.CDLL(path)
+    for s in ("dropin_plan_f32_3d", "dropin_t3_f64_3d", "dropin_t3_f32_2d", "dropin_input_error"):
+        assert hasattr(lib, s), s

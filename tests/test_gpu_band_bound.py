@@ -1,0 +1,120 @@
+"""Row-band spreading (2D type 1) with strengths far from uniform in size.
+
+A row band accumulates in fixed point with one scale per band. Its bound on
+a cell is the smaller of (points of the cell's window x the transform's
+largest |Re c| + |Im c|) and (the sum of |Re c| + |Im c| over the points of
+the cell's ns-row window): a few strengths far above the rest must not
+coarsen every band's scale. The reference spreads in floating point
+(src/neonufft/gpu/kernels/spreading_kernel.cu:527-586), so its precision does
+not depend on the strengths' spread; these cases check that ours stays
+close to the subgrid-tile path: ten bright sources, and points in a disk with
+weights proportional to the radius (density-compensation-like). Each is compared with the FP64 direct
+NUDFT, with the FP64 restatement of the algorithm (oracle/), and with the
+subgrid-tile path of the same plan (whose scale is per subproblem).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import neonufft_amd as nnf
+from oracle import oracle as O
+
+from test_gpu_plan import sample_modes
+
+pytestmark = pytest.mark.gpu
+
+MODES = (256, 256)
+TOL = 1e-5
+
+
+def _strengths(kind, rng, x, y):
+    m = len(x)
+    c = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    if kind == "bright":
+        # ten sources 300x brighter than the rest
+        k = rng.choice(m, size=10, replace=False)
+        c[k] *= 300.0
+    elif kind == "radial":
+        c *= np.hypot(x.astype(np.float64), y.astype(np.float64))
+    return c.astype(np.complex64)
+
+
+def _points(kind, rng, m):
+    if kind == "radial":
+        r = np.pi * np.sqrt(rng.uniform(0.0, 1.0, m))
+        a = rng.uniform(-np.pi, np.pi, m)
+        return (r * np.cos(a)).astype(np.float32), (r * np.sin(a)).astype(np.float32)
+    return (rng.uniform(-np.pi, np.pi, m).astype(np.float32),
+            rng.uniform(-np.pi, np.pi, m).astype(np.float32))
+
+
+def _run(plan, c, n):
+    out = torch.zeros(n, dtype=torch.complex64, device="cuda")
+    plan.transform_type_1(torch.from_numpy(c).cuda(), out)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().astype(np.complex128)
+
+
+@pytest.mark.parametrize("kind", ["bright", "radial", "uniform"])
+def test_band_bound_strength_spread(kind):
+    rng = np.random.default_rng({"bright": 11, "radial": 12, "uniform": 13}[kind])
+    m = 1_000_000
+    x, y = _points(kind, rng, m)
+    c = _strengths(kind, rng, x, y)
+    loc = [torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()]
+    opt = nnf.Options(tol=TOL)
+    n = int(np.prod(MODES))
+    band = nnf.Plan(opt, -1, m, loc, MODES)
+    assert band.info()["spread_mode"] == "bands"
+    tiles = nnf.Plan(opt, -1, m, loc, MODES, spread_mode="tiles")
+    got_b = _run(band, c, n)
+    got_t = _run(tiles, c, n)
+    xs = [x.astype(np.float64), y.astype(np.float64)]
+    c64 = c.astype(np.complex128)
+    alg = O.nufft1(xs, c64, MODES, -1, TOL)
+    # fp32 share of the error: distance to the same algorithm in FP64
+    e_band = O.rel_l2(got_b, alg)
+    e_tile = O.rel_l2(got_t, alg)
+    print(f"{kind}: band {e_band:.3e} tiles {e_tile:.3e} vs FP64 restatement", flush=True)
+    assert e_band < max(2.0 * e_tile, 2e-6), (e_band, e_tile)
+    idx = sample_modes(MODES, rng, 300)
+    ref = O.direct_type1(xs, c64, O.mode_vectors(MODES, idx), -1)
+    alg_err = O.rel_l2(alg[idx], ref)
+    err = O.rel_l2(got_b[idx], ref)
+    assert alg_err < TOL
+    print(f"{kind}: direct {err:.3e} (FP64 restatement {alg_err:.3e})", flush=True)
+    assert err < TOL and err < alg_err + 3 * e_band + 1e-6, (err, alg_err, e_band)
+
+
+def test_band_bound_batched():
+    """The batched row-band path takes the same bound per transform: a bright
+    transform next to a plain one keeps both at the precision of their own
+    plain plans."""
+    rng = np.random.default_rng(21)
+    specs = [("bright", 600_000), ("uniform", 300_000), ("radial", 400_000)]
+    locs, cs, hosts = [], [], []
+    for kind, m in specs:
+        x, y = _points(kind, rng, m)
+        c = _strengths(kind, rng, x, y)
+        hosts.append((x, y, c))
+        locs.append([torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()])
+        cs.append(torch.from_numpy(c).cuda())
+    opt = nnf.Options(tol=TOL)
+    n = int(np.prod(MODES))
+    batch = nnf.BatchPlan(opt, 1, [m for _, m in specs], locs, [list(MODES)] * len(specs),
+                          use_graph=False)
+    assert batch.info()["spread_mode"] == "bands"
+    outs = [torch.zeros(n, dtype=torch.complex64, device="cuda") for _ in specs]
+    batch.transform_type_1(cs, outs)
+    torch.cuda.synchronize()
+    for t, (kind, m) in enumerate(specs):
+        x, y, c = hosts[t]
+        got = outs[t].cpu().numpy().astype(np.complex128)
+        alg = O.nufft1([x.astype(np.float64), y.astype(np.float64)], c.astype(np.complex128),
+                       MODES, 1, TOL)
+        tiles = nnf.Plan(opt, 1, m, locs[t], MODES, spread_mode="tiles")
+        e_tile = O.rel_l2(_run(tiles, c, n), alg)
+        e_band = O.rel_l2(got, alg)
+        print(f"batched {kind}: band {e_band:.3e} tiles {e_tile:.3e}", flush=True)
+        assert e_band < max(2.0 * e_tile, 2e-6), (kind, e_band, e_tile)
