@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(kRowBlock)
     row_scatter_kernel(const PointRec<T, 2>* __restrict__ recs, const uint32_t* __restrict__ perm,
                        std::size_t m, BinGeom g, int ns, const uint32_t* __restrict__ row_start,
                        const uint32_t* __restrict__ rank, PointRec<T, 2>* __restrict__ out,
-                       uint32_t* __restrict__ perm_out) {
+                       uint32_t* __restrict__ perm_out, uint16_t* __restrict__ row_out) {
   for (std::size_t p = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kRowBlock; p < m;
        p += static_cast<std::size_t>(gridDim.x) * kRowBlock) {
     PointRec<T, 2> r = recs[p];
@@ -75,6 +75,7 @@ __global__ void __launch_bounds__(kRowBlock)
     const uint32_t pos = row_start[row] + rank[p];
     out[pos] = r;
     perm_out[pos] = perm[p];
+    row_out[pos] = static_cast<uint16_t>(row);
   }
 }
 
@@ -200,31 +201,11 @@ __global__ void __launch_bounds__(kRowBlock)
   for (int k = 0; k < kGatherItems; ++k) {
     if (dst[k]) *dst[k] = v[k];
   }
-  // First kernel row of every item: the last row whose view offset is <= p
-  // (binary search over the job's n1 + 1 row offsets, all items in step).
-  int lo[kGatherItems], hi[kGatherItems];
-  const uint32_t* rs[kGatherItems];
+  // first kernel row of every item (row table of the view)
+  int lo[kGatherItems];
 #pragma unroll
   for (int k = 0; k < kGatherItems; ++k) {
-    const BandJob<T>& jb = jobs ? jobs[jk[k]] : single;
-    rs[k] = jb.row_start;
-    lo[k] = 0;
-    hi[k] = dst[k] ? jb.n1 : 1;
-  }
-  for (bool more = true; more;) {
-    more = false;
-#pragma unroll
-    for (int k = 0; k < kGatherItems; ++k) {
-      if (hi[k] - lo[k] > 1) {
-        const int mid = (lo[k] + hi[k]) >> 1;
-        if (rs[k][mid] <= pp[k]) {
-          lo[k] = mid;
-        } else {
-          hi[k] = mid;
-        }
-        more = more || hi[k] - lo[k] > 1;
-      }
-    }
+    lo[k] = dst[k] ? static_cast<int>((jobs ? jobs[jk[k]] : single).vrow[pp[k]]) : 0;
   }
   const int lane = static_cast<int>(threadIdx.x & 63);
 #pragma unroll
@@ -757,10 +738,12 @@ void launch_row_count(const PointRec<T, 2>* recs, std::size_t m, const BinGeom& 
 template <typename T>
 void launch_row_scatter(const PointRec<T, 2>* recs, const uint32_t* perm, std::size_t m,
                         const BinGeom& g, int ns, const uint32_t* row_start, const uint32_t* rank,
-                        PointRec<T, 2>* out, uint32_t* perm_out, hipStream_t stream) {
+                        PointRec<T, 2>* out, uint32_t* perm_out, uint16_t* row_out,
+                        hipStream_t stream) {
   if (m == 0) return;
+  if (g.n[1] > 65536) throw neonufft::InternalError("row view needs at most 65536 rows");
   hipLaunchKernelGGL((row_scatter_kernel<T>), dim3(blocks_for(m, kRowBlock)), dim3(kRowBlock), 0,
-                     stream, recs, perm, m, g, ns, row_start, rank, out, perm_out);
+                     stream, recs, perm, m, g, ns, row_start, rank, out, perm_out, row_out);
   check_launch("row_scatter_kernel");
 }
 
@@ -841,7 +824,7 @@ std::size_t band_lds_bytes(int rows, int pitch, std::size_t acc_bytes) {
                                     uint32_t*, uint32_t*, hipStream_t);                         \
   template void launch_row_scatter<T>(const PointRec<T, 2>*, const uint32_t*, std::size_t,      \
                                       const BinGeom&, int, const uint32_t*, const uint32_t*,    \
-                                      PointRec<T, 2>*, uint32_t*, hipStream_t);                 \
+                                      PointRec<T, 2>*, uint32_t*, uint16_t*, hipStream_t);      \
   template void launch_row_block_count<T>(const PointRec<T, 2>*, std::size_t, int, int,          \
                                           uint32_t*, hipStream_t);                               \
   template void launch_band_gather<T>(const BandJob<T>*, const BandJob<T>&, int, std::size_t,   \

@@ -300,6 +300,7 @@ public:
           j.pts = rows_[t].recs();
           j.perm = rows_[t].perm();
           j.row_start = rows_[t].row_start();
+          j.vrow = rows_[t].vrow();
           j.bcount = rows_[t].block_counts();
           j.rowsum = reinterpret_cast<T*>(bounds_.data() + it.bound_off);
           j.rowmax = j.rowsum + g.n[1];

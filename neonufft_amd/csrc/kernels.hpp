@@ -164,14 +164,16 @@ std::size_t tile_bytes(const BinGeom& g);
 // ---- band.hip: row-band spreading (2D type 1) ------------------------------
 // Row-sorted view of a tile-sorted 2D point set: records sorted by their first
 // kernel row i0y (mod n1), with cell = (i0x + ns/2) | (row << 16) and
-// bin = row; perm_out maps the view back to the caller's indices.
+// bin = row; perm_out maps the view back to the caller's indices and
+// row_out[p] is the first kernel row of view point p (n1 <= 65536).
 template <typename T>
 void launch_row_count(const PointRec<T, 2>* recs, std::size_t m, const BinGeom& g, int ns,
                       uint32_t* count, uint32_t* rank, hipStream_t stream);
 template <typename T>
 void launch_row_scatter(const PointRec<T, 2>* recs, const uint32_t* perm, std::size_t m,
                         const BinGeom& g, int ns, const uint32_t* row_start, const uint32_t* rank,
-                        PointRec<T, 2>* out, uint32_t* perm_out, hipStream_t stream);
+                        PointRec<T, 2>* out, uint32_t* perm_out, uint16_t* row_out,
+                        hipStream_t stream);
 
 // One 2D transform of a row-band launch. Band b of the launch belongs to the
 // job with the largest band_off <= b and covers rows
@@ -181,6 +183,7 @@ struct BandJob {
   const PointRec<T, 2>* pts;   // row-sorted records
   const uint32_t* perm;        // row-sorted position -> caller's point index
   const uint32_t* row_start;   // n1 + 1 offsets into the view
+  const uint16_t* vrow;        // first kernel row of each view point
   const uint32_t* bcount;      // n1 x kBoundBlocks: points per (first row, column block)
   T* rowsum;                   // n1: sum of |Re c| + |Im c| over the points of each first row
   T* rowmax;                   // n1: largest |Re c| + |Im c| among them

@@ -317,6 +317,7 @@ private:
     j.pts = rows_.recs();
     j.perm = rows_.perm();
     j.row_start = rows_.row_start();
+    j.vrow = rows_.vrow();
     j.bcount = rows_.block_counts();
     j.rowsum = reinterpret_cast<T*>(bounds_.data());
     j.rowmax = j.rowsum + n_[1];

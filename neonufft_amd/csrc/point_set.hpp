@@ -106,6 +106,7 @@ public:
     rank_.ensure(m ? m : 1, alloc);
     recs_.ensure(m ? m : 1, alloc);
     perm_.ensure(m ? m : 1, alloc);
+    vrow_.ensure(m ? m : 1, alloc);
     // upper bound on the chunks: one partial chunk per row plus the full ones
     const std::size_t max_chunks = n1 + m / kMaxSubproblemPoints + 1;
     chunks_.ensure(max_chunks, alloc);
@@ -116,7 +117,7 @@ public:
     launch_row_count<T>(ps.recs(), m, g, ns, count_.data(), rank_.data(), stream);
     launch_exclusive_scan(count_.data(), start_.data(), n1 + 1, scratch_.data(), stream);
     launch_row_scatter<T>(ps.recs(), ps.perm(), m, g, ns, start_.data(), rank_.data(),
-                          recs_.data(), perm_.data(), stream);
+                          recs_.data(), perm_.data(), vrow_.data(), stream);
     launch_subprob_counts(count_.data(), n1 + 1, kMaxSubproblemPoints, nchunk_.data(), stream);
     launch_exclusive_scan(nchunk_.data(), chunk_start_.data(), n1 + 1, scratch_.data(), stream);
     launch_subprob_build(count_.data(), start_.data(), chunk_start_.data(), n1,
@@ -139,6 +140,8 @@ public:
   const PointRec<T, 2>* recs() const { return recs_.data(); }
   const uint32_t* perm() const { return perm_.data(); }
   const uint32_t* row_start() const { return start_.data(); }
+  // first kernel row of each view point (bank order keeps points in their row)
+  const uint16_t* vrow() const { return vrow_.data(); }
   const uint32_t* block_counts() const { return bcount_.data(); }
   // device: the largest point count any cell's fixed-point bound sees
   const uint32_t* window_max() const { return wmax_.data(); }
@@ -149,6 +152,7 @@ private:
   DeviceBuffer<PointRec<T, 2>> recs_;
   DeviceBuffer<uint32_t> perm_, rank_, count_, start_, nchunk_, chunk_start_, scratch_, bcount_,
       wmax_;
+  DeviceBuffer<uint16_t> vrow_;
   DeviceBuffer<Subprob> chunks_;
 };
 
