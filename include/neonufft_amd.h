@@ -77,6 +77,23 @@ int nnf_plan_create(nnf_plan* plan, int precision, int dim, const nnf_options* o
 /* gpu::Plan<T,DIM>::set_points (include/neonufft/gpu/plan.hpp:42). */
 int nnf_plan_set_points(nnf_plan plan, int64_t num_nu, const void* const* loc);
 
+/* No counterpart in the reference: nnf_plan_set_points without any device
+ * allocation, for callers that cannot allocate (serving loops that must not
+ * call hipMalloc between requests, allocation-free sections of a program).
+ * The temporaries of the fold and the counting sorts live in a fixed array
+ * inside the library, 64 MiB per GPU (held from the library's load to the
+ * process's end), which limits a call to about 16 million points; the
+ * results go to the memory that earlier nnf_plan_create / nnf_plan_set_points
+ * calls left in the plan. Fails with NEONUFFT_INPUT_ERROR, before any work,
+ * when num_nu exceeds the largest point count the plan has held or the
+ * temporaries exceed the array; fails the same way, after the sort, when the
+ * new points need more subproblems or spreading tiles than the plan holds
+ * (clustered points after uniform ones), and the plan then refuses
+ * transforms until a set_points succeeds. All plans on one GPU share its
+ * array: calls are serialised per device and return with the plan's stream
+ * synchronised, so the array is free again when a call returns. */
+int nnf_plan_set_points_noalloc(nnf_plan plan, int64_t num_nu, const void* const* loc);
+
 /* gpu::Plan<T,DIM>::transform_type_1 (include/neonufft/gpu/plan.hpp:34-35):
  * in = num_nu complex strengths, out = modes written with out_strides. */
 int nnf_plan_transform_type_1(nnf_plan plan, const void* in, void* out,

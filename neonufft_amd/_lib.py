@@ -10,7 +10,8 @@ LIB_PATH = os.path.join(HERE, "libneonufft_amd.so")
 # Every symbol the C header declares (checked by the CPU test suite).
 EXPORTED = (
     "nnf_options_default", "nnf_last_error", "nnf_version",
-    "nnf_plan_create", "nnf_plan_set_points", "nnf_plan_transform_type_1",
+    "nnf_plan_create", "nnf_plan_set_points", "nnf_plan_set_points_noalloc",
+    "nnf_plan_transform_type_1",
     "nnf_plan_transform_type_2", "nnf_plan_destroy",
     "nnf_plan_t3_grid_memory_size", "nnf_plan_t3_create", "nnf_plan_t3_create_bounds",
     "nnf_plan_t3_set_input_points", "nnf_plan_t3_set_output_points", "nnf_plan_t3_add_input",
@@ -113,6 +114,7 @@ def lib():
     L.nnf_plan_create_tuned.argtypes = [vpp, c_int, c_int, optp, c_int, i64, vpp, i64p, vp,
                                         ctypes.POINTER(NnfTuning)]
     L.nnf_plan_set_points.argtypes = [vp, i64, vpp]
+    L.nnf_plan_set_points_noalloc.argtypes = [vp, i64, vpp]
     L.nnf_plan_transform_type_1.argtypes = [vp, vp, vp, i64p]
     L.nnf_plan_transform_type_2.argtypes = [vp, vp, i64p, vp]
     L.nnf_plan_destroy.argtypes = [vp]

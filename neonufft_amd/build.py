@@ -41,6 +41,7 @@ SOURCES = [
     ("band.hip", True),
     ("grid.hip", True),
     ("pruned.hip", True),
+    ("noalloc.hip", True),
     ("params.cpp", False),
     ("fft.cpp", False),
     ("plan.cpp", False),

@@ -28,6 +28,7 @@ thread_local std::string g_last_error;
 struct PlanIface {
   virtual ~PlanIface() = default;
   virtual void set_points(int64_t n, const void* const* loc) = 0;
+  virtual void set_points_noalloc(int64_t n, const void* const* loc) = 0;
   virtual void t1(const void* in, void* out, const int64_t* st) = 0;
   virtual void t2(const void* in, const int64_t* st, void* out) = 0;
   virtual void spread(const void* in, bool accumulate) = 0;
@@ -48,6 +49,9 @@ struct PlanHolder final : PlanIface {
       : impl(o, sign, m, reinterpret_cast<const T* const*>(loc), modes, s, std::move(a), t) {}
   void set_points(int64_t n, const void* const* loc) override {
     impl.set_points(n, reinterpret_cast<const T* const*>(loc));
+  }
+  void set_points_noalloc(int64_t n, const void* const* loc) override {
+    impl.set_points_noalloc(n, reinterpret_cast<const T* const*>(loc));
   }
   void t1(const void* in, void* out, const int64_t* st) override {
     impl.transform_type_1(static_cast<const Cplx<T>*>(in), static_cast<Cplx<T>*>(out), st);
@@ -305,6 +309,14 @@ NEONUFFT_EXPORT int nnf_plan_set_points(nnf_plan plan, int64_t num_nu, const voi
   return guarded([&] {
     NNF_REQUIRE_PLAN(plan);
     plan->p->set_points(num_nu, loc);
+  });
+}
+
+NEONUFFT_EXPORT int nnf_plan_set_points_noalloc(nnf_plan plan, int64_t num_nu,
+                                                const void* const* loc) {
+  return guarded([&] {
+    NNF_REQUIRE_PLAN(plan);
+    plan->p->set_points_noalloc(num_nu, loc);
   });
 }
 

@@ -89,6 +89,26 @@ private:
   std::shared_ptr<neonufft::Allocator> alloc_;
 };
 
+// Temporary device memory carved out of a caller-provided region (no
+// allocation): take() hands out 256-byte aligned pieces in order. With a null
+// base it only counts the bytes (a dry run that sizes the region).
+struct Workspace {
+  char* base = nullptr;
+  std::size_t cap = 0;
+  std::size_t used = 0;
+
+  template <typename E>
+  E* take(std::size_t count) {
+    const std::size_t off = (used + 255) & ~std::size_t(255);
+    const std::size_t bytes = count * sizeof(E);
+    if (base && off + bytes > cap) {
+      throw neonufft::InputError("Neonufft:: workspace too small");
+    }
+    used = off + bytes;
+    return base ? reinterpret_cast<E*>(base + off) : nullptr;
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Device-side complex pair and per-precision helpers.
 // ---------------------------------------------------------------------------

@@ -43,6 +43,9 @@ std::size_t scan_scratch_size(std::size_t n);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, std::size_t n, uint32_t* scratch,
                            hipStream_t stream);
 
+// p[0..n) = v (a kernel, not hipMemsetAsync, so graphs may record it).
+void launch_fill_u32(uint32_t* p, std::size_t n, uint32_t v, hipStream_t stream);
+
 void launch_subprob_counts(const uint32_t* count, std::size_t nbins, uint32_t max_points,
                            uint32_t* nsub, hipStream_t stream);
 void launch_subprob_build(const uint32_t* count, const uint32_t* bin_start,

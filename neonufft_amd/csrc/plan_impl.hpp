@@ -18,6 +18,7 @@
 #include <neonufft_amd.h>
 #include "cover.hpp"
 #include "fft.hpp"
+#include "noalloc.hpp"
 #include "params.hpp"
 #include "point_set.hpp"
 #include "pruned_fft.hpp"
@@ -134,6 +135,7 @@ public:
            const PlanTuning& tune = PlanTuning())
       : opt_(opt), sign_(sign), stream_(stream), alloc_(std::move(alloc)), tune_(tune) {
     if (!alloc_) throw neonufft::InputError("Neonufft:: null allocator");
+    check_hip(hipGetDevice(&device_), "hipGetDevice");
     ks_ = make_kernel_setup(opt_.tol, opt_.upsampfac, std::is_same_v<T, float>);
     for (int d = 0; d < 3; ++d) {
       modes_[d] = 1;
@@ -169,18 +171,44 @@ public:
     set_points(num_nu, loc);
   }
 
-  void set_points(IntType num_nu, const T* const* loc) {
+  // ws: null = buffers grow as needed; otherwise nothing is allocated (the
+  // temporaries come from *ws, the results must fit the memory an earlier
+  // allocating set_points left, else InputError). Until a set_points
+  // succeeds, the plan refuses transforms.
+  void set_points(IntType num_nu, const T* const* loc, Workspace* ws = nullptr) {
     if (num_nu < 0) throw neonufft::InputError("Neonufft:: negative number of points");
     Loc<T> l{{nullptr, nullptr, nullptr}};
     for (int d = 0; d < DIM; ++d) l.p[d] = loc ? loc[d] : nullptr;
     FoldParams fp{{0, 0, 0}, {1, 1, 1}};
-    pts_.build(l, static_cast<std::size_t>(num_nu), fp, geom_, ks_.ns, tune_.max_points, stream_,
-               alloc_);
+    const std::size_t m = static_cast<std::size_t>(num_nu);
+    if (ws) {
+      // everything this call takes from ws, checked before any work
+      std::size_t need = PointSet<T, DIM>::scratch_bytes(
+          m, static_cast<std::size_t>(geom_.nbins[0]) * geom_.nbins[1] * geom_.nbins[2]);
+      if constexpr (DIM == 2) {
+        if (band_rows_ > 0) {
+          need = std::max(need, RowView<T>::scratch_bytes(m, static_cast<std::size_t>(n_[1])));
+          if (sorted_.size() < std::max<std::size_t>(m, 1)) {
+            throw neonufft::InputError(
+                "Neonufft:: more points than the plan holds memory for (set_points without "
+                "allocation)");
+          }
+        }
+      }
+      if (need > ws->cap) {
+        throw neonufft::InputError(
+            "Neonufft:: set_points without allocation needs more scratch memory than the "
+            "library's fixed maximum");
+      }
+    }
+    points_valid_ = false;
+    pts_.build(l, m, fp, geom_, ks_.ns, tune_.max_points, stream_, alloc_, ws);
     bands_on_ = false;
     if constexpr (DIM == 2) {
       if (band_rows_ > 0) {
-        rows_.build(pts_, ks_.ns, stream_, alloc_);
-        sorted_.ensure(std::max<std::size_t>(pts_.size(), 1), alloc_);
+        if (ws) ws->used = 0;  // the point set's temporaries are no longer needed
+        rows_.build(pts_, ks_.ns, stream_, alloc_, ws);
+        if (!ws) sorted_.ensure(std::max<std::size_t>(pts_.size(), 1), alloc_);
         row_host_.resize(static_cast<std::size_t>(n_[1]) + 2);
         check_hip(hipMemcpyAsync(row_host_.data(), rows_.row_start(),
                                  (row_host_.size() - 1) * sizeof(uint32_t),
@@ -198,7 +226,37 @@ public:
                                  row_host_[static_cast<std::size_t>(n_[1]) + 1],
                                  tune_.spread_mode == kSpreadAuto);
     }
-    if (!bands_on_) tiles_.ensure(std::max<std::size_t>(pts_.nsub(), 1) * tile_cells(geom_), alloc_);
+    if (!bands_on_) {
+      const std::size_t need = std::max<std::size_t>(pts_.nsub(), 1) * tile_cells(geom_);
+      if (ws) {
+        if (tiles_.size() < need) {
+          throw neonufft::InputError(
+              "Neonufft:: the points need more spreading tiles than the plan holds memory for "
+              "(set_points without allocation)");
+        }
+      } else if (tiles_.size() < need) {
+        // 1/16 headroom, so that point sets of the same size with a few more
+        // subproblems fit without a new allocation
+        const std::size_t nsub = std::max<std::size_t>(pts_.nsub(), 1);
+        tiles_.reset((nsub + nsub / 16 + 1) * tile_cells(geom_), alloc_);
+      }
+    }
+    points_valid_ = true;
+  }
+
+  // set_points with no device allocation: temporaries in the device's fixed
+  // scratch array (held for the whole call, which ends with the stream
+  // synchronised), results in the memory of earlier set_points calls.
+  void set_points_noalloc(IntType num_nu, const T* const* loc) {
+    ScratchLease lease = lease_noalloc_scratch(device_);
+    try {
+      set_points(num_nu, loc, &lease.ws);
+    } catch (...) {
+      // work already enqueued may still read the array: finish it before
+      // another call can take the array
+      (void)hipStreamSynchronize(stream_);
+      throw;
+    }
   }
 
   // Stage 1 of type 1: form the fine grid from the strengths (overwriting
@@ -212,6 +270,7 @@ public:
 
   // The two kernels of spread(), separately (for stage timing).
   void spread_tiles(const Cplx<T>* in) {
+    require_points();
     if (pts_.size() && !in) throw neonufft::InputError("Neonufft:: null input");
     if constexpr (DIM == 2) {
       if (bands_on_) {
@@ -224,6 +283,7 @@ public:
                           tiles_.data(), geom_, ks_.es, stream_);
   }
   void reduce_tiles(bool accumulate) {
+    require_points();
     if constexpr (DIM == 2) {
       if (bands_on_) {
         const int nb = static_cast<int>((n_[1] + band_rows_ - 1) / band_rows_);
@@ -240,12 +300,14 @@ public:
 
   // Stage 3 of type 2: interpolate the current fine grid.
   void interp(Cplx<T>* out) {
+    require_points();
     if (pts_.size() && !out) throw neonufft::InputError("Neonufft:: null output");
     launch_interp<T, DIM>(ks_.ns, pts_.recs(), pts_.perm(), pts_.subs(), pts_.nsub(), grid_.data(),
                           nullptr, out, geom_, ks_.es, stream_);
   }
 
   void transform_type_1(const Cplx<T>* in, Cplx<T>* out, const IntType* out_strides) {
+    require_points();
     if (!out || !out_strides) throw neonufft::InputError("Neonufft:: null output");
     T* fs[3] = {fs_[0].data(), fs_[1].data(), fs_[2].data()};
     if constexpr (DIM == 2) {
@@ -274,6 +336,7 @@ public:
   }
 
   void transform_type_2(const Cplx<T>* in, const IntType* in_strides, Cplx<T>* out) {
+    require_points();
     if (!in || !in_strides) throw neonufft::InputError("Neonufft:: null input");
     T* fs[3] = {fs_[0].data(), fs_[1].data(), fs_[2].data()};
     if constexpr (DIM == 1) {
@@ -312,6 +375,12 @@ public:
 private:
   int order_fft() const { return opt_.order == NEONUFFT_MODE_ORDER_FFT ? 1 : 0; }
 
+  void require_points() const {
+    if (!points_valid_) {
+      throw neonufft::InputError("Neonufft:: no valid points (the last set_points failed)");
+    }
+  }
+
   BandJob<T> band_job(const Cplx<T>* in) const {
     BandJob<T> j{};
     j.pts = rows_.recs();
@@ -345,12 +414,14 @@ private:
   hipStream_t stream_;
   std::shared_ptr<neonufft::Allocator> alloc_;
   PlanTuning tune_;
+  int device_ = 0;                // device current at construction
   KernelSetup ks_;
   IntType modes_[3];
   IntType n_[3];
   BinGeom geom_{};
   int band_rows_ = 0;             // 2D row-band spreading (0 = subgrid tiles)
   bool bands_on_ = false;         // row bands used for the current points
+  bool points_valid_ = false;     // the last set_points succeeded
   std::vector<uint32_t> row_host_;  // row offsets of the view (balance check)
   BandLayout band_;
   DeviceBuffer<Cplx<T>> tw_;      // x-FFT twiddles of the fused type 1

@@ -176,6 +176,14 @@ __global__ void __launch_bounds__(kScanThreads)
 }
 
 __global__ void __launch_bounds__(kBlock)
+    fill_u32_kernel(uint32_t* __restrict__ p, std::size_t n, uint32_t v) {
+  for (std::size_t i = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock; i < n;
+       i += static_cast<std::size_t>(gridDim.x) * kBlock) {
+    p[i] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
     subprob_counts_kernel(const uint32_t* __restrict__ count, std::size_t nbins,
                           uint32_t max_points, uint32_t* __restrict__ nsub) {
   for (std::size_t b = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kBlock; b < nbins;
@@ -356,6 +364,12 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, std::size_t n, uin
   hipLaunchKernelGGL(scan_tiles_kernel, dim3(static_cast<unsigned>(tiles)), dim3(kScanThreads), 0,
                      stream, in, n, scratch, out);
   check_launch("scan_tiles_kernel");
+}
+
+void launch_fill_u32(uint32_t* p, std::size_t n, uint32_t v, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, stream, p, n, v);
+  check_launch("fill_u32_kernel");
 }
 
 void launch_subprob_counts(const uint32_t* count, std::size_t nbins, uint32_t max_points,
