@@ -8,14 +8,21 @@ plus one type-2 transform (modes -> M values) on a plan whose points are set;
 inputs are resident in HBM. value = NU points processed per second
 (2 M per step) over the whole job.
 
-Prints ONE JSON line. Extra keys: the spreading / interpolation kernel
-rooflines (HIP events on the plan's stream), a bounded CPU baseline (the FP64
-C restatement in oracle/, OpenMP), the relative l2 error of both
-transforms on 1000 sampled outputs against the FP64 direct NUDFT, and the
-batched-uneven comparison: 256 independent 2D type-1 transforms (complex64,
-point counts log-uniform in [1e4, 1e6], modes drawn from {256^2, 512^2,
-1024^2}, tol 1e-5) through one BatchPlan (one spreading launch, batched FFTs,
-HIP graph) against 256 sequential plain plans on the same GPU.
+Prints ONE JSON line. Extra keys: the roofline of the slower of the spread
+stage (spreading + tile reduction) and interpolation (HIP events on the plan's
+stream) against HBM and against LDS, with the rocprofv3 counters committed
+under profiles/; a bounded CPU baseline (the FP64 C restatement in oracle/,
+OpenMP) with the host's core count and CPU model; the relative l2 error of
+both transforms on 1000 sampled outputs against the FP64 direct NUDFT, beside
+the error of the same algorithm in FP64 on the same samples; and three more
+workloads, each with its own accuracy and CPU baseline:
+  batched: 256 independent 2D type-1 transforms (complex64, point counts
+    log-uniform in [1e4, 1e6], modes drawn from {256^2, 512^2, 1024^2}, tol
+    1e-5) through one BatchPlan (HIP graph) against 256 plain plans;
+  clustered: 3D type 2, 2e7 points in 16 Gaussian clusters, 256^3 modes, tol
+    1e-4 and 1e-6, autotuned tiles;
+  type3: 3D type 3, 2e6 sources (|x| up to 1e4), 1e6 targets, tol 1e-6, in
+    the mixed, fp32 and fp64 plans.
 """
 from __future__ import annotations
 
@@ -30,6 +37,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# LDS: 128 bytes per clock per CU, 256 CUs, 2.4 GHz peak engine clock
+LDS_PEAK = 128 * 256 * 2.4e9
 SEED = 0x4E554654
 
 
@@ -55,9 +64,46 @@ def parse():
     p.add_argument("--batch-count", type=int, default=256,
                    help="transforms in the batched-uneven comparison (0 = skip it)")
     p.add_argument("--batch-reps", type=int, default=5)
+    p.add_argument("--no-clustered", action="store_true",
+                   help="leave the clustered type-2 leg out of the default line")
+    p.add_argument("--no-type3", action="store_true",
+                   help="leave the type-3 leg out of the default line")
     p.add_argument("--calib-copy-mib", type=int, default=0,
                    help="also run a plain device copy of this many MiB (profiler calibration)")
     return p.parse_args()
+
+
+def host_cpu():
+    """Threads the CPU legs use and what the host offers: the logical CPUs this
+    process may run on, the physical cores (distinct (package, core) pairs in
+    /proc/cpuinfo) and the CPU model."""
+    from oracle import oracle as O
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = os.cpu_count()
+    return {"cores": O.num_threads(), "physical_cores_host": len(cores) or None,
+            "logical_cpus_visible": visible, "cpu_model": model}
 
 
 def cpu_baseline_leg(x, c, f_gpu, out2_gpu, modes, tol, sample):
@@ -77,6 +123,21 @@ def cpu_baseline_leg(x, c, f_gpu, out2_gpu, modes, tol, sample):
     pidx = np.sort(rng.choice(m, size=min(1000, m), replace=False))
     ref2 = O.direct_type2([v[pidx] for v in xs], f_gpu.astype(np.complex128), modes, -1)
     err2 = O.rel_l2(out2_gpu[pidx], ref2)
+    # the same algorithm in FP64 (oracle/) on the same samples: the ns-wide
+    # kernel's own error, which the reference shares; the rest of the GPU
+    # error is fp32 rounding (Horner fit, fixed-point sums, fp32 FFT)
+    t_alg = time.perf_counter()
+    alg1 = O.nufft1(xs, c.astype(np.complex128), modes, -1, tol)
+    alg2 = O.nufft2([v[pidx] for v in xs], f_gpu.astype(np.complex128), modes, -1, tol)
+    acc = {
+        "type1_rel_l2": err1, "type2_rel_l2": err2,
+        "fp64_restatement_type1_rel_l2": O.rel_l2(alg1[idx], ref1),
+        "fp64_restatement_type2_rel_l2": O.rel_l2(alg2, ref2),
+        "fp32_share_type1": O.rel_l2(f_gpu[idx], alg1[idx]),
+        "fp32_share_type2": O.rel_l2(out2_gpu[pidx], alg2),
+        "samples": 1000, "reference": "FP64 direct NUDFT",
+        "restatement_s": time.perf_counter() - t_alg,
+    }
 
     # timed CPU restatement on the first `sample` points of the same workload
     ms = min(sample, m)
@@ -90,13 +151,13 @@ def cpu_baseline_leg(x, c, f_gpu, out2_gpu, modes, tol, sample):
     cpu = {
         "value": 2.0 * ms / (t2 - t0),
         "unit": "NU pts/s",
-        "cores": O.num_threads(),
+        **host_cpu(),
         "kind": "port",
         "sample": (f"FP64 C restatement (oracle/nufft_oracle.c, OpenMP) + numpy FFT, "
                    f"type 1 + type 2 on the first {ms} points of the workload, same grid; "
                    f"type1 {t1 - t0:.2f}s type2 {t2 - t1:.2f}s"),
     }
-    return cpu, err1, err2
+    return cpu, acc
 
 
 def batched_leg(args, dev, stream):
@@ -246,6 +307,23 @@ def _timed(stream, fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
+PROFILE_COUNTERS = "profiles/counters_3d_c64_M1e7_modes128.json"
+
+
+def _profile_counters():
+    """Per-dispatch rocprofv3 counters of the default workload's kernels
+    (HBM bytes from FETCH_SIZE / WRITE_SIZE scaled by a copy of known size,
+    LDS busy and bank-conflict fractions), committed under profiles/."""
+    path = os.path.join(ROOT, PROFILE_COUNTERS)
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            return json.load(fh)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def _common(args, metric, value, ms_step, dtype, config):
     return {
         "metric": metric, "value": value, "unit": "NU pts/s", "n_gpus": 1,
@@ -294,14 +372,16 @@ def cpu_leg_type2(x, f, out_gpu, modes, tol, sample, nacc):
     t0 = time.perf_counter()
     O.nufft2([v[:ms] for v in xs], f64, modes, -1, tol)
     t1 = time.perf_counter()
-    cpu = {"value": ms / (t1 - t0), "unit": "NU pts/s", "cores": O.num_threads(),
+    cpu = {"value": ms / (t1 - t0), "unit": "NU pts/s", **host_cpu(),
            "kind": "port",
            "sample": (f"FP64 C restatement (oracle/, OpenMP) + numpy FFT, type 2 on the first "
                       f"{ms} points of the workload, same grid: {t1 - t0:.2f}s")}
     return cpu, err
 
 
-def run_clustered(args):
+def clustered_leg(args):
+    """cfg3-style workload: 3D type 2 on 2e7 clustered points, 256^3 modes,
+    tol 1e-4 and 1e-6, tiles chosen by the autotuner."""
     import numpy as np
     import torch
 
@@ -369,20 +449,30 @@ def run_clustered(args):
             r["accuracy_samples"] = 200
             if cpu is None:
                 cpu = c
-    head = results[0]
+    del dx, df, out
+    torch.cuda.synchronize()
+    return {"workload": "3D type-2, complex64, M=2e7 points in 16 Gaussian clusters "
+                        "(sigma 0.05, folded), modes 256^3 (fine grid 512^3), "
+                        "tol 1e-4 and 1e-6, autotuned tiles",
+            "points": m, "modes": list(modes), "sweep": results, "cpu_baseline": cpu}
+
+
+def run_clustered(args):
+    leg = clustered_leg(args)
+    head = leg["sweep"][0]
     line = _common(args, "NU pts/s, 3D type-2 fp32, 2e7 clustered points, 256^3 modes, 1 GPU",
                    head["pts_per_s"], head["ms_per_step"], "f32",
-                   {"workload": "3D type-2, complex64, M=2e7 points in 16 Gaussian clusters "
-                                "(sigma 0.05, folded), modes 256^3 (fine grid 512^3), "
-                                "tol 1e-4 and 1e-6, autotuned tiles",
-                    "points": m, "modes": list(modes)})
+                   {"workload": leg["workload"], "points": leg["points"],
+                    "modes": leg["modes"]})
     line["roofline"] = head["roofline"]
-    line["sweep"] = results
-    line["cpu_baseline"] = cpu
+    line["sweep"] = leg["sweep"]
+    line["cpu_baseline"] = leg["cpu_baseline"]
     print(json.dumps(line), flush=True)
 
 
-def run_type3(args):
+def type3_leg(args):
+    """cfg4-style workload: 3D type 3, 2e6 sources, 1e6 targets, tol 1e-6, in
+    the mixed, fp32 and fp64 plans."""
     import numpy as np
     import torch
 
@@ -459,21 +549,28 @@ def run_type3(args):
         t0 = time.perf_counter()
         O.nufft3([v[:ms_] for v in x], c[:ms_], [v[:ns_] for v in s], -1, 1e-6)
         t1 = time.perf_counter()
-        cpu = {"value": (ms_ + ns_) / (t1 - t0), "unit": "NU pts/s", "cores": O.num_threads(),
+        cpu = {"value": (ms_ + ns_) / (t1 - t0), "unit": "NU pts/s", **host_cpu(),
                "kind": "port",
                "sample": (f"FP64 C restatement (oracle/, OpenMP) + numpy FFT, type 3 with the "
                           f"first {ms_} sources and {ns_} targets: {t1 - t0:.2f}s")}
+    return {"workload": "3D type-3, M=2e6 sources x,y in U[-1e4,1e4], z in U[-50,50]; "
+                        "N=1e6 targets s_x,s_y in U[-0.02,0.02], s_z in U[-1,1]; "
+                        "tol 1e-6; mixed plan (double coordinates, complex128 I/O, "
+                        "FP64 fold and phases, fp32 spreading / FFT / interpolation); "
+                        "fp32 and fp64 plans alongside",
+            "sources": m, "targets": n, "precisions": res, "cpu_baseline": cpu}
+
+
+def run_type3(args):
+    leg = type3_leg(args)
+    mixed = leg["precisions"]["mixed"]
     line = _common(args, "NU pts/s (sources + targets), 3D type-3 @1e-6, 1 GPU",
-                   res["mixed"]["pts_per_s"], res["mixed"]["ms_per_step"], "f32",
-                   {"workload": "3D type-3, M=2e6 sources x,y in U[-1e4,1e4], z in U[-50,50]; "
-                                "N=1e6 targets s_x,s_y in U[-0.02,0.02], s_z in U[-1,1]; "
-                                "tol 1e-6; mixed plan (double coordinates, complex128 I/O, "
-                                "FP64 fold and phases, fp32 spreading / FFT / interpolation); "
-                                "fp32 and fp64 plans alongside",
-                    "sources": m, "targets": n})
-    line["roofline"] = res["mixed"]["roofline"]
-    line["precisions"] = res
-    line["cpu_baseline"] = cpu
+                   mixed["pts_per_s"], mixed["ms_per_step"], "f32",
+                   {"workload": leg["workload"], "sources": leg["sources"],
+                    "targets": leg["targets"]})
+    line["roofline"] = mixed["roofline"]
+    line["precisions"] = leg["precisions"]
+    line["cpu_baseline"] = leg["cpu_baseline"]
     print(json.dumps(line), flush=True)
 
 
@@ -554,8 +651,9 @@ def run_uniform(args):
     reps = args.kernel_reps
     t1_ms = timed(lambda: plan.transform_type_1(dc, f), reps)
     t2_ms = timed(lambda: plan.transform_type_2(f, None, out2), reps)
-    # the kernels alone: spreading into subgrid tiles, the tile reduction that
-    # forms the fine grid, and interpolation
+    # the spread stage (strengths -> fine grid: spreading into subgrid tiles
+    # plus the tile reduction), its two kernels alone, and interpolation
+    stage_ms = timed(lambda: plan.spread(dc), reps)
     spread_ms = timed(lambda: plan.spread_tiles(dc), reps)
     reduce_ms = timed(lambda: plan.reduce_tiles(accumulate=True), reps)
     interp_ms = timed(lambda: plan.interp(out2), reps)
@@ -576,40 +674,63 @@ def run_uniform(args):
     G = 1
     for n in info["grid"]:
         G *= n
+    ns = info["n_spread"]
     alg_bytes = m * (3 * 4 + 8) + G * 8  # coords + strength, fine grid once
-    spread_gbs = alg_bytes / (spread_ms * 1e-3)
-    interp_gbs = alg_bytes / (interp_ms * 1e-3)
     # bytes the reduction moves: every stored tile cell read once, grid written
     tile_cells = 1
     for d, nb in enumerate(info["bin_size"]):
-        tile_cells *= nb + info["n_spread"]
+        tile_cells *= nb + ns
     reduce_bytes = info["num_subproblems"] * tile_cells * 8 + G * 8
-    dom = ("spread_kernel", spread_ms, spread_gbs) if spread_ms >= interp_ms else \
-        ("interp_kernel", interp_ms, interp_gbs)
+    # LDS work of the algorithm: spreading adds re and im (two 4-byte integer
+    # atomics) into each of ns^3 cells per point; interpolation reads each of
+    # ns^3 cells per point (one 8-byte complex read)
+    lds_spread = m * ns ** 3 * 2 * 4
+    lds_interp = m * ns ** 3 * 8
+    prof = _profile_counters()
+    legs = {
+        "spread_stage": {"ms": stage_ms, "lds_bytes": lds_spread,
+                         "kernels": ["spread_kernel", "bin_reduce_kernel"]},
+        "interp_kernel": {"ms": interp_ms, "lds_bytes": lds_interp, "kernels": ["interp_kernel"]},
+    }
+    dom_name = max(legs, key=lambda k: legs[k]["ms"])
+    dom = legs[dom_name]
+    traffic = None
+    if prof:
+        try:
+            traffic = sum(prof[k]["hbm_bytes_per_launch"] for k in dom["kernels"])
+        except KeyError:
+            traffic = None
+    hbm_gbs = alg_bytes / (dom["ms"] * 1e-3)
+    lds_bps = dom["lds_bytes"] / (dom["ms"] * 1e-3)
+    lds = {"achieved": lds_bps / 1e12, "peak": LDS_PEAK / 1e12, "unit": "TB/s",
+           "frac": lds_bps / LDS_PEAK, "lds_bytes_per_launch": dom["lds_bytes"],
+           "note": ("algorithmic LDS data bytes (spreading: 2 x 4-byte atomics per point and "
+                    "cell, each a read-modify-write of its bank; interpolation: one 8-byte "
+                    "read per point and cell) / launch time, against 128 B/clk/CU x 256 CUs "
+                    "x 2.4 GHz")}
+    if prof:
+        busy = {}
+        for k in dom["kernels"]:
+            c = prof.get(k, {})
+            if "lds_busy_frac" in c:
+                busy[k] = {"lds_busy_frac": c["lds_busy_frac"],
+                           "bank_conflict_frac_of_busy": c.get("bank_conflict_frac_of_busy")}
+        lds["rocprof"] = busy
+    bound = "lds" if lds_bps / LDS_PEAK > hbm_gbs / HBM_PEAK else "hbm"
 
     f_host = f.cpu().numpy()
     out2_host = out2.cpu().numpy()
 
     cpu = None
-    err1 = err2 = None
+    acc = {"samples": 1000, "reference": "FP64 direct NUDFT"}
     if not args.no_cpu_baseline:
-        cpu, err1, err2 = cpu_baseline_leg(x, c, f_host, out2_host, modes, args.tol,
-                                           args.cpu_sample)
+        cpu, acc = cpu_baseline_leg(x, c, f_host, out2_host, modes, args.tol, args.cpu_sample)
 
-    batched = None
-    if args.batch_count > 0:
-        del plan
-        torch.cuda.synchronize()
-        batched = batched_leg(args, dev, stream)
-
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tfile):
-        try:
-            tj = json.load(open(tfile))
-            traffic = tj.get(dom[0], {}).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    del plan
+    torch.cuda.synchronize()
+    batched = batched_leg(args, dev, stream) if args.batch_count > 0 else None
+    clustered = None if args.no_clustered else clustered_leg(args)
+    type3 = None if args.no_type3 else type3_leg(args)
 
     line = {
         "metric": "NU pts/s, 3D type-1/2 fp32 @1e-5 vs fp64 oracle; batched vs plain, 1 GPU",
@@ -628,35 +749,40 @@ def run_uniform(args):
             "workload": f"3D type-1 + type-2, complex64, M={m}, modes {N}^3, tol {args.tol:g}, "
                         f"upsampfac 2.0, fine grid {'x'.join(map(str, info['grid']))}",
             "points": m, "modes": [N, N, N], "tol": args.tol, "upsampfac": 2.0,
-            "n_spread": info["n_spread"], "bin_size": info["bin_size"],
+            "n_spread": ns, "bin_size": info["bin_size"],
             "subproblems": info["num_subproblems"],
         },
         "roofline": {
-            "bound": "hbm",
-            "kernel": dom[0],
-            "achieved": dom[2] / 1e9,
+            "bound": bound,
+            "kernel": dom_name,
+            "achieved": hbm_gbs / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
-            "frac": dom[2] / HBM_PEAK,
+            "frac": hbm_gbs / HBM_PEAK,
             "traffic": traffic,
+            "traffic_source": PROFILE_COUNTERS if traffic is not None else None,
             "alg_bytes_per_launch": alg_bytes,
-            "avg_launch_ms": dom[1],
+            "avg_launch_ms": dom["ms"],
+            "lds": lds,
         },
         "kernels": {
-            "spread_ms": spread_ms, "spread_GBps": spread_gbs / 1e9,
+            "spread_stage_ms": stage_ms,
+            "spread_ms": spread_ms, "spread_GBps": alg_bytes / (spread_ms * 1e-3) / 1e9,
             "reduce_ms": reduce_ms, "reduce_GBps": reduce_bytes / (reduce_ms * 1e-3) / 1e9,
             "reduce_bytes": reduce_bytes,
-            "spread_stage_ms": spread_ms + reduce_ms,
-            "interp_ms": interp_ms, "interp_GBps": interp_gbs / 1e9,
+            "interp_ms": interp_ms, "interp_GBps": alg_bytes / (interp_ms * 1e-3) / 1e9,
+            "interp_lds_frac": lds_interp / (interp_ms * 1e-3) / LDS_PEAK,
+            "spread_stage_lds_frac": lds_spread / (stage_ms * 1e-3) / LDS_PEAK,
         },
         "stages": {
             "type1_ms": t1_ms, "type2_ms": t2_ms, "set_points_ms": set_points_ms,
             "plan_create_ms": create_ms,
             "type1_pts_per_s": m / (t1_ms * 1e-3), "type2_pts_per_s": m / (t2_ms * 1e-3),
         },
-        "accuracy": {"type1_rel_l2": err1, "type2_rel_l2": err2,
-                     "samples": 1000, "reference": "FP64 direct NUDFT"},
+        "accuracy": acc,
         "batched": batched,
+        "clustered": clustered,
+        "type3": type3,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -6,7 +6,8 @@ transform gives what a separate plan of the reference's interface
 (include/neonufft/gpu/plan.hpp:25-46) gives for the same points, modes and
 options. So every transform is compared with a plain ``Plan`` run alone
 (same kernels, so agreement is to rounding of the float atomics' order), and
-one transform per case with the FP64 direct sum (closed-form truth).
+with the FP64 direct sum (closed-form truth) and the same algorithm in FP64
+(oracle/), type 1 and type 2.
 Graph replays are checked against eager runs with changing buffers and
 unrelated eager work between the replays.
 """
@@ -18,7 +19,8 @@ torch = pytest.importorskip("torch")
 import neonufft_amd as nnf
 from oracle import oracle as O
 
-from test_gpu_plan import CDT, RDT, bound, cplx, dev, points
+from test_gpu_layouts import to_cmcl
+from test_gpu_plan import CDT, RDT, alg_bound, amplification, bound, cplx, dev, points
 
 pytestmark = pytest.mark.gpu
 
@@ -102,15 +104,29 @@ def test_batch_matches_plans(dim, prec, order, sign):
             assert _rel(_to_np(outs1[t]), _to_np(r1)) < SAME[prec], f"type 1 transform {t}"
             assert _rel(_to_np(outs2[t])[:m], _to_np(r2)[:m]) < SAME[prec], f"type 2 transform {t}"
 
-    # one transform against the FP64 direct sum
-    t = max(range(len(spec)), key=lambda k: spec[k][1])
-    md, m = spec[t]
-    xs = [_to_np(v).astype(np.float64) for v in locs[t]]
-    kk = O.mode_vectors(md, np.arange(int(np.prod(md))),
-                        order_fft=nnf.ModeOrder(order) == nnf.ModeOrder.FFT)
-    truth1 = O.direct_type1(xs, _to_np(cs[t]).astype(np.complex128), kk, sign)
-    err = _rel(_to_np(outs1[t]).astype(np.complex128), truth1)
-    assert err < bound(prec, tol, 1) * 20, f"type 1 vs direct: {err}"
+    # every transform, type 1 and type 2, against the FP64 direct sum and the
+    # same algorithm in FP64 (oracle/)
+    offt = nnf.ModeOrder(order) == nnf.ModeOrder.FFT
+    for t, (md, m) in enumerate(spec):
+        if m == 0:
+            continue
+        xs = [_to_np(v).astype(np.float64) for v in locs[t]]
+        amp = amplification(md, tol, 2.0)
+        c64 = _to_np(cs[t]).astype(np.complex128)
+        kk = O.mode_vectors(md, np.arange(int(np.prod(md))), order_fft=offt)
+        truth1 = O.direct_type1(xs, c64, kk, sign)
+        alg1 = O.nufft1(xs, c64, md, sign, tol, order_fft=offt)
+        got1 = _to_np(outs1[t]).astype(np.complex128)
+        alg_err = _rel(alg1, truth1)
+        assert _rel(got1, truth1) < bound(prec, alg_err, tol, amp), f"type 1 transform {t}"
+        assert _rel(got1, alg1) < alg_bound(prec, tol, amp), f"type 1 transform {t}"
+        f64 = _to_np(fs[t]).astype(np.complex128)
+        truth2 = O.direct_type2(xs, to_cmcl(f64, md, offt), md, sign)
+        alg2 = O.nufft2(xs, f64, md, sign, tol, order_fft=offt)
+        got2 = _to_np(outs2[t])[:m].astype(np.complex128)
+        alg_err = _rel(alg2, truth2)
+        assert _rel(got2, truth2) < bound(prec, alg_err, tol, amp), f"type 2 transform {t}"
+        assert _rel(got2, alg2) < alg_bound(prec, tol, amp), f"type 2 transform {t}"
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])

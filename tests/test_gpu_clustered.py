@@ -84,7 +84,7 @@ def test_clustered_type2(data, tol, tuned):
     amp = amplification(MODES, tol, 2.0)
     assert alg_err < 20 * tol
     err = O.rel_l2(got[pidx], ref)
-    assert err < bound(prec, alg_err, 2, amp), (err, alg_err)
+    assert err < bound(prec, alg_err, tol, amp), (err, alg_err)
     assert O.rel_l2(got[pidx], alg) < alg_bound(prec, tol, amp)
 
 
@@ -113,7 +113,7 @@ def test_clustered_type1(data, tuned, prec):
     amp = amplification(MODES, tol, 2.0)
     assert alg_err < 20 * tol
     err = O.rel_l2(got[midx], ref)
-    assert err < bound(prec, alg_err, 1, amp), (err, alg_err)
+    assert err < bound(prec, alg_err, tol, amp), (err, alg_err)
     assert O.rel_l2(got, alg) < alg_bound(prec, tol, amp)
 
 

@@ -123,7 +123,7 @@ def test_type2_layouts(case, kind, order, prec):
     amp = amplification(modes, tol, 2.0)
     assert alg_err < 20 * tol
     err = O.rel_l2(got, ref)
-    assert err < bound(prec, alg_err, 2, amp), (err, alg_err)
+    assert err < bound(prec, alg_err, tol, amp), (err, alg_err)
     assert O.rel_l2(got, alg) < alg_bound(prec, tol, amp)
 
 
@@ -161,7 +161,7 @@ def test_type1_layouts(case, kind, order, prec):
     amp = amplification(modes, tol, 2.0)
     assert alg_err < 20 * tol
     err = O.rel_l2(got, ref)
-    assert err < bound(prec, alg_err, 1, amp), (err, alg_err)
+    assert err < bound(prec, alg_err, tol, amp), (err, alg_err)
     assert O.rel_l2(got, alg) < alg_bound(prec, tol, amp)
 
 

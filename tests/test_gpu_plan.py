@@ -52,26 +52,28 @@ def amplification(modes, tol, sigma):
     return a
 
 
-def bound(prec, alg_err, kind, amp=1.0):
+def bound(prec, alg_err, tol, amp=1.0):
     """Allowed relative l2 error against the FP64 direct sum.
 
     alg_err is the error of the same algorithm run in FP64 (oracle/): the
     kernel truncation error that the reference's (tol, upsampfac) -> (ns, beta)
-    rule leaves (es_kernel_param.hpp:24-56). The plan may add rounding: ~1e-12
-    in fp64, and in fp32 a floor of ~1e-5 (type 2) to ~2e-5 (type 1) plus the
-    fp32 rounding amplified by the deconvolution (amp)."""
-    if prec == "f64":
-        return 1.2 * alg_err + 1e-12
-    return 1.2 * alg_err + (2e-5 if kind == 1 else 1e-5) + 2e-8 * amp
+    rule leaves (es_kernel_param.hpp:24-56). By the triangle inequality the
+    plan's error is at most alg_err plus its distance to that FP64 result,
+    which alg_bound caps (1.2 alg_err: the sampled subset)."""
+    return 1.2 * alg_err + alg_bound(prec, tol, amp)
 
 
 def alg_bound(prec, tol, amp):
-    """Allowed relative l2 distance to the same algorithm run in FP64: fp64
-    rounding, or fp32 rounding (amplified by the deconvolution) plus the
-    fp32 Horner fit of the ES kernel (~1e-2 tol at ns = 4, far less beyond)."""
+    """Allowed relative l2 distance to the same algorithm run in FP64 (the
+    fp32 share of the error): fp64 rounding; or fp32 rounding, 3e-6, plus the
+    same rounding amplified by the deconvolution 1/phihat (amp: up to ~4e3 at
+    sigma = 1.25), plus the fp32 Horner fit of the ES kernel (~0.02 tol at
+    ns = 4, far less beyond). Measured on MI355X over the type 1 / 2 parity
+    cases: at most 1e-6 at sigma = 2 and tol <= 1e-4, 3.2e-5 at sigma = 1.25
+    (amp 4180), 2.2e-5 at tol 1e-3 (ns = 4)."""
     if prec == "f64":
         return 1e-10
-    return 2e-5 + 2e-8 * amp + 0.05 * tol
+    return 3e-6 + 1.5e-8 * amp + 0.04 * tol
 
 
 def sample_modes(modes, rng, s=400):
@@ -135,7 +137,7 @@ def test_type1(case, prec):
     alg_err = O.rel_l2(alg[idx], ref)
     amp = amplification(modes, tol, sigma)
     assert alg_err < 20 * tol
-    assert err < bound(prec, alg_err, 1, amp), (err, alg_err, amp)
+    assert err < bound(prec, alg_err, tol, amp), (err, alg_err, amp)
     # same algorithm in FP64: only Horner / fp32 rounding separate them
     assert O.rel_l2(got, alg) < alg_bound(prec, tol, amp)
 
@@ -162,7 +164,7 @@ def test_type2(case, prec):
     alg_err = O.rel_l2(alg, ref)
     amp = amplification(modes, tol, sigma)
     assert alg_err < 20 * tol
-    assert err < bound(prec, alg_err, 2, amp), (err, alg_err, amp)
+    assert err < bound(prec, alg_err, tol, amp), (err, alg_err, amp)
     assert O.rel_l2(got, alg) < alg_bound(prec, tol, amp)
 
 
@@ -306,7 +308,7 @@ def test_type1_spread_modes(case, order, prec):
     alg = O.nufft1(xs, c.astype(np.complex128), modes, sign, tol, 2.0, order_fft=offt)
     alg_err = O.rel_l2(alg[idx], ref)
     for mode, got in outs.items():
-        assert O.rel_l2(got[idx], ref) < bound(prec, alg_err, 1, amp), mode
+        assert O.rel_l2(got[idx], ref) < bound(prec, alg_err, tol, amp), mode
     same = 1e-12 if prec == "f64" else 5e-6
     assert O.rel_l2(outs["bands"], outs["tiles"]) < same
     assert O.rel_l2(outs["bands_grid"], outs["tiles"]) < same
