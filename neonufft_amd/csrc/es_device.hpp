@@ -22,6 +22,22 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t n) {
 
 __device__ inline int wrap_once(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// i = q d + r with 0 <= r < d for 0 <= i < 2^22 and 0 < d < 2^16, from a float
+// reciprocal rd = 1/d: the product i * rd carries a relative error below
+// 2^-23, i.e. less than (i / d) 2^-23 < 1 / (2 d) absolute, which is less than
+// the distance from i / d up to the next integer; so its truncation is q or
+// q - 1, fixed by one correction. Returns r, sets *q.
+__device__ inline int div_rem(int i, int d, float rd, int* q) {
+  int t = static_cast<int>(static_cast<float>(i) * rd);
+  int r = i - t * d;
+  if (r >= d) {
+    ++t;
+    r -= d;
+  }
+  *q = t;
+  return r;
+}
+
 // Weight of kernel piece j (cell i0 + j) for Horner argument x in [0,1):
 // phi(x + j - NS/2).
 template <int NS>

@@ -209,20 +209,40 @@ __device__ __forceinline__ void interp_subproblem(const InterpJob<T, DIM>& job,
   const BinGeom& g = job.g;
   const TileInfo<DIM, NS> ti(sp.bin, g);
 
-  // Stage the tile (bin + halo) from the grid with coalesced row reads.
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int rows = ti.ext[1] * ti.ext[2];
-  for (int r = wave; r < rows; r += kInterpThreads<T> / 64) {
-    const int i1 = r % ti.ext[1];
-    const int i2 = r / ti.ext[1];
-    const int gy = DIM >= 2 ? wrap_once(ti.org[1] + i1, g.n[1]) : 0;
-    const int gz = DIM >= 3 ? wrap_once(ti.org[2] + i2, g.n[2]) : 0;
-    const std::size_t grow =
-        (static_cast<std::size_t>(gz) * static_cast<std::size_t>(g.n[1]) + gy) * g.n[0];
-    const int trow = (i2 * ti.t1 + i1) * ti.t0;
-    for (int i0 = lane; i0 < ti.ext[0]; i0 += 64) {
-      tile[trow + i0] = grid[grow + wrap_once(ti.org[0] + i0, g.n[0])];
+  // Stage the tile (bin + halo) from the grid. The loop runs over the tile's
+  // cells flattened (x fastest, so reads stay coalesced along rows) and each
+  // lane issues kStage independent loads before storing any: one 38-cell row
+  // per wave and step left 40 % of the lanes idle and exposed the full load
+  // latency once per row.
+  {
+    constexpr int kStage = 4;
+    const int e0 = ti.ext[0], e1 = ti.ext[1];
+    const int ncell = e0 * e1 * ti.ext[2];
+    const float r0 = 1.0f / static_cast<float>(e0), r1 = 1.0f / static_cast<float>(e1);
+    for (int base = threadIdx.x; base < ncell; base += kInterpThreads<T> * kStage) {
+      Cplx<T> v[kStage];
+      int dst[kStage];
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        const int i = base + k * kInterpThreads<T>;
+        dst[k] = -1;
+        if (i < ncell) {
+          int i1, i2;
+          const int i0 = div_rem(i, e0, r0, &i1);
+          i1 = div_rem(i1, e1, r1, &i2);
+          const int gx = wrap_once(ti.org[0] + i0, g.n[0]);
+          const int gy = DIM >= 2 ? wrap_once(ti.org[1] + i1, g.n[1]) : 0;
+          const int gz = DIM >= 3 ? wrap_once(ti.org[2] + i2, g.n[2]) : 0;
+          v[k] = grid[(static_cast<std::size_t>(gz) * static_cast<std::size_t>(g.n[1]) + gy) *
+                          static_cast<std::size_t>(g.n[0]) +
+                      gx];
+          dst[k] = (i2 * ti.t1 + i1) * ti.t0 + i0;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        if (dst[k] >= 0) tile[dst[k]] = v[k];
+      }
     }
   }
   __syncthreads();
