@@ -186,8 +186,15 @@ int nnf_plan_create_tuned(nnf_plan* plan, int precision, int dim, const nnf_opti
  * with row bands time their type 1 for a list of band heights instead of
  * the tile shapes (best->band_rows). `reps` timed calls per
  * candidate. With use_cache non-zero, a result is reused for the same
- * precision, dim, kernel width, fine grid, floor(log2 num_nu) and types in
- * this process. */
+ * precision, dim, kernel width, fine grid, floor(log2 num_nu), types and
+ * point-distribution signature (largest over mean occupied-bin count and share
+ * of empty bins on the default tiles, so clustered and uniform points of one
+ * size are tuned separately), from this process or from the cache file
+ * written by earlier processes: NEONUFFT_AMD_TUNE_CACHE names it ("" = no
+ * file), else $XDG_CACHE_HOME or $HOME/.cache /neonufft_amd/
+ * autotune-<version>-<gfx arch>.txt. best->reserved[0] tells where the
+ * answer came from: 0 = timed by this call, 1 = this process's cache,
+ * 2 = the cache file. */
 int nnf_autotune(int precision, int dim, const nnf_options* opt, int sign, int64_t num_nu,
                  const void* const* loc, const int64_t* modes, void* stream, int types, int reps,
                  int use_cache, nnf_tuning* best, double* best_ms);

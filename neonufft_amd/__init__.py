@@ -160,8 +160,9 @@ def autotune(opt: "Options", sign: int, num_nu: int, loc: Sequence[Pointer],
     arguments, timed on these device points (extension; no reference
     counterpart). types: 1 = spreading, 2 = interpolation, 3 = both. 2D plans
     spreading with row bands also get the fastest band height (type 1 timed).
-    Returns {"bin_size": [...], "max_points": int, "band_rows": int, "ms": float}
-    (band_rows 0 = default)."""
+    Returns {"bin_size": [...], "max_points": int, "band_rows": int, "ms": float,
+    "source": int} (band_rows 0 = default; source 0 = timed by this call, 1 =
+    this process's cache, 2 = the cache file of earlier processes)."""
     L = _L.lib()
     dim = len(modes)
     prec = _dtype_code(dtype) if dtype is not None else _precision_of(loc[0])
@@ -172,7 +173,8 @@ def autotune(opt: "Options", sign: int, num_nu: int, loc: Sequence[Pointer],
                           _i64p(modes), ctypes.c_void_p(_stream(stream)), int(types), int(reps),
                           int(bool(use_cache)), ctypes.byref(best), ctypes.byref(ms)))
     return {"bin_size": list(best.bin_size)[:dim], "max_points": int(best.max_points),
-            "band_rows": int(best.band_rows), "ms": float(ms.value)}
+            "band_rows": int(best.band_rows), "ms": float(ms.value),
+            "source": int(best.reserved[0])}
 
 
 class Plan:

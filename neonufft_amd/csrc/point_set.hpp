@@ -3,6 +3,7 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -94,6 +95,18 @@ public:
                                hipMemcpyDeviceToDevice, stream),
                 "hipMemcpyAsync(subproblem starts)");
     }
+  }
+
+  // Points per bin (nbins values) after an allocating build; synchronises.
+  std::vector<uint32_t> host_bin_counts(hipStream_t stream) const {
+    std::vector<uint32_t> h(nbins_);
+    if (nbins_) {
+      check_hip(hipMemcpyAsync(h.data(), count_.data(), nbins_ * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, stream),
+                "hipMemcpyAsync(bin counts)");
+      check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+    return h;
   }
 
   // Workspace bytes a build without allocation takes for m points over nbins bins.
