@@ -1,7 +1,8 @@
 #pragma once
 /* Static configuration of the MI355X (gfx950) NeoNUFFT drop-in.
  * Replaces the CMake-generated config.h of the reference (cmake/config.h.in:1-11):
- * this build has exactly one GPU backend, ROCm/HIP, and no CPU plan. */
+ * this build has exactly one backend, ROCm/HIP: the host-memory plans of
+ * <neonufft/plan.hpp> also run on the GPU. */
 
 #define NEONUFFT_VERSION "0.1.0-mi355x"
 #define NEONUFFT_ROCM

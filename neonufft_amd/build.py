@@ -45,6 +45,7 @@ SOURCES = [
     ("params.cpp", False),
     ("fft.cpp", False),
     ("plan.cpp", False),
+    ("host_plan.cpp", False),
     ("autotune.cpp", False),
     ("c_api.cpp", False),
 ]

@@ -8,9 +8,11 @@
 //    calls them through ctypes and checks the results against the oracle.
 #include <neonufft/exceptions.hpp>
 #include <neonufft/gpu/plan.hpp>
+#include <neonufft/plan.hpp>
 
 #include <hip/hip_runtime_api.h>
 
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -111,6 +113,62 @@ int dropin_t3_f32_2d(int64_t m, const float* const* x, int64_t n, const float* c
                                              2);
         plan.add_input(static_cast<const hipComplex*>(c), m + 1);
         plan.transform(static_cast<hipComplex*>(out), n + 1);
+      },
+      msg, msglen);
+}
+
+// Host-memory neonufft::Plan<double, 2> (include/neonufft/plan.hpp:31-52 of
+// the reference): every pointer a host pointer. Type 1 of c into f1 written
+// with out_strides st1, type 2 of f_in read with in_strides st2 into c_out;
+// then set_points to B and type 1 of c2 into f2 (contiguous).
+int dropin_host_plan_f64_2d(int64_t ma, const double* const* xa, int64_t mb,
+                            const double* const* xb, const int64_t* modes, double tol, int sign,
+                            const void* c1, void* f1, const int64_t* st1, const void* f_in,
+                            const int64_t* st2, void* c_out, const void* c2, void* f2,
+                            char* msg, int msglen) {
+  return guarded(
+      [&] {
+        neonufft::Options opt;
+        opt.tol = tol;
+        using C = std::complex<double>;
+        neonufft::Plan<double, 2> plan(opt, sign, ma, {xa[0], xa[1]}, {modes[0], modes[1]});
+        plan.transform_type_1(static_cast<const C*>(c1), static_cast<C*>(f1), {st1[0], st1[1]});
+        plan.transform_type_2(static_cast<const C*>(f_in), {st2[0], st2[1]},
+                              static_cast<C*>(c_out));
+        plan.set_points(mb, {xb[0], xb[1]});
+        plan.transform_type_1(static_cast<const C*>(c2), static_cast<C*>(f2), {1, modes[0]});
+      },
+      msg, msglen);
+}
+
+// Host-memory neonufft::PlanT3<float, 3> from bounds, batch of 2 with bdist:
+// two add_input calls (c1, c2; inputs at in_bdist) then transform into out1
+// (at out_bdist); reset, one add_input (c3), transform into out2. Also the
+// reference CPU plan's grid_memory_size for the same bounds.
+int dropin_host_t3_f32_3d(int64_t m, const float* const* x, int64_t n, const float* const* s,
+                          const float* bounds, double tol, int sign, const void* c1,
+                          const void* c2, const void* c3, int64_t in_bdist, int64_t out_bdist,
+                          void* out1, void* out2, uint64_t* grid_bytes, char* msg, int msglen) {
+  return guarded(
+      [&] {
+        neonufft::Options opt;
+        opt.tol = tol;
+        using C = std::complex<float>;
+        const std::array<float, 3> in_min{bounds[0], bounds[1], bounds[2]};
+        const std::array<float, 3> in_max{bounds[3], bounds[4], bounds[5]};
+        const std::array<float, 3> out_min{bounds[6], bounds[7], bounds[8]};
+        const std::array<float, 3> out_max{bounds[9], bounds[10], bounds[11]};
+        *grid_bytes =
+            neonufft::PlanT3<float, 3>::grid_memory_size(opt, in_min, in_max, out_min, out_max);
+        neonufft::PlanT3<float, 3> plan(opt, sign, in_min, in_max, out_min, out_max, 2);
+        plan.set_input_points(m, {x[0], x[1], x[2]});
+        plan.set_output_points(n, {s[0], s[1], s[2]});
+        plan.add_input(static_cast<const C*>(c1), in_bdist);
+        plan.add_input(static_cast<const C*>(c2), in_bdist);
+        plan.transform(static_cast<C*>(out1), out_bdist);
+        plan.reset();
+        plan.add_input(static_cast<const C*>(c3), in_bdist);
+        plan.transform(static_cast<C*>(out2), out_bdist);
       },
       msg, msglen);
 }

@@ -160,3 +160,107 @@ def test_dropin_input_error(hooks):
     msg = _msg()
     rc = hooks.dropin_input_error(msg, 512)
     assert rc == 5, (rc, msg.value.decode())  # NEONUFFT_INPUT_ERROR
+
+
+def _host_ptrs(arrs):
+    return (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def _hp(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def test_dropin_host_plan_f64_2d(hooks):
+    """neonufft::Plan<double, 2> on host arrays (the reference's CPU plan
+    interface): strided type-1 output and type-2 input, then set_points."""
+    rng = np.random.default_rng(74)
+    modes = (24, 17)
+    nm = modes[0] * modes[1]
+    ma, mb, tol, sign = 900, 500, 1e-9, 1
+    xa = [rng.uniform(-9.0, 4.0, ma) for _ in range(2)]
+    xb = [rng.uniform(-np.pi, np.pi, mb) for _ in range(2)]
+    c1, c2 = cplx(rng, ma, "f64"), cplx(rng, mb, "f64")
+    fin = cplx(rng, nm, "f64")
+    # type-1 output padded (rows of 27), type-2 input transposed (y fastest)
+    st1, st2 = (1, 27), (17, 1)
+    off1 = np.array([(k % 24) * st1[0] + (k // 24) * st1[1] for k in range(nm)])
+    off2 = np.array([(k % 24) * st2[0] + (k // 24) * st2[1] for k in range(nm)])
+    sentinel = 7.0 - 5.0j
+    f1 = np.full(int(off1.max()) + 1, sentinel, dtype=np.complex128)
+    fin_buf = np.zeros(nm, dtype=np.complex128)
+    fin_buf[off2] = fin
+    cout = np.zeros(ma, dtype=np.complex128)
+    f2 = np.zeros(nm, dtype=np.complex128)
+    md = (ctypes.c_int64 * 2)(*modes)
+    s1 = (ctypes.c_int64 * 2)(*st1)
+    s2 = (ctypes.c_int64 * 2)(*st2)
+    msg = _msg()
+    rc = hooks.dropin_host_plan_f64_2d(ctypes.c_int64(ma), _host_ptrs(xa), ctypes.c_int64(mb),
+                                       _host_ptrs(xb), md, ctypes.c_double(tol), sign, _hp(c1),
+                                       _hp(f1), s1, _hp(fin_buf), s2, _hp(cout), _hp(c2), _hp(f2),
+                                       msg, 512)
+    assert rc == 0, msg.value.decode()
+    gaps = np.ones(f1.size, dtype=bool)
+    gaps[off1] = False
+    assert np.all(f1[gaps] == sentinel), "wrote outside the strided positions"
+    kk = O.mode_vectors(modes, np.arange(nm))
+    for x, c, got in ((xa, c1, f1[off1]), (xb, c2, f2)):
+        ref = O.direct_type1(x, c, kk, sign)
+        alg = O.nufft1(x, c, modes, sign, tol)
+        assert O.rel_l2(alg, ref) < 20 * tol
+        assert O.rel_l2(got, ref) < 1.2 * O.rel_l2(alg, ref) + 1e-12
+        assert O.rel_l2(got, alg) < 1e-10
+    ref2 = O.direct_type2(xa, fin, modes, sign)
+    alg2 = O.nufft2(xa, fin, modes, sign, tol)
+    assert O.rel_l2(cout, ref2) < 1.2 * O.rel_l2(alg2, ref2) + 1e-12
+    assert O.rel_l2(cout, alg2) < 1e-10
+
+
+def test_dropin_host_t3_f32_3d(hooks):
+    """neonufft::PlanT3<float, 3> on host arrays from bounds: batch of two
+    with gaps between the members, two add_input calls accumulating, reset;
+    and grid_memory_size against the reference CPU plan's formula."""
+    rng = np.random.default_rng(75)
+    tol, sign, m, n = 1e-5, -1, 700, 400
+    in_lo, in_hi = [-3.0, 1.0, -8.0], [4.0, 6.0, -2.0]
+    out_lo, out_hi = [-5.0, -2.0, 0.5], [2.0, 3.0, 6.0]
+    x = [rng.uniform(in_lo[d], in_hi[d], m).astype(np.float32) for d in range(3)]
+    s = [rng.uniform(out_lo[d], out_hi[d], n).astype(np.float32) for d in range(3)]
+    bi, bo = m + 3, n + 2
+    cs = [[cplx(rng, m, "f32") for _ in range(2)] for _ in range(3)]
+    bufs = []
+    for call in cs:
+        b = np.full(2 * bi, 1e6 + 1e6j, dtype=np.complex64)
+        b[:m], b[bi:bi + m] = call[0], call[1]
+        bufs.append(b)
+    out1 = np.full(2 * bo, 9.0 + 0j, dtype=np.complex64)
+    out2 = np.full(2 * bo, 9.0 + 0j, dtype=np.complex64)
+    bounds = np.array(in_lo + in_hi + out_lo + out_hi, dtype=np.float32)
+    gbytes = ctypes.c_uint64()
+    msg = _msg()
+    rc = hooks.dropin_host_t3_f32_3d(ctypes.c_int64(m), _host_ptrs(x), ctypes.c_int64(n),
+                                     _host_ptrs(s), _hp(bounds), ctypes.c_double(tol), sign,
+                                     _hp(bufs[0]), _hp(bufs[1]), _hp(bufs[2]),
+                                     ctypes.c_int64(bi), ctypes.c_int64(bo), _hp(out1), _hp(out2),
+                                     ctypes.byref(gbytes), msg, 512)
+    assert rc == 0, msg.value.decode()
+    ns, _ = O.kernel_params(tol, 2.0)
+    b64 = [bounds[0:3].astype(np.float64), bounds[3:6].astype(np.float64),
+           bounds[6:9].astype(np.float64), bounds[9:12].astype(np.float64)]
+    gi = O.GridInfoT3.make(ns, 2.0, 0.1, *b64)
+    pad = 2 * (ns // 2 + 2)
+    assert gbytes.value == 8 * (int(np.prod([v + pad for v in gi.spread_grid_size]))
+                                + int(np.prod(gi.fft_grid_size)))
+    for o in (out1, out2):
+        assert o[n] == 9.0 and o[bo + n] == 9.0, "wrote into the gap between batch members"
+    xs = [v.astype(np.float64) for v in x]
+    ss = [v.astype(np.float64) for v in s]
+    for b in range(2):
+        for got, c in ((out1[b * bo:b * bo + n], cs[0][b].astype(np.complex128)
+                        + cs[1][b].astype(np.complex128)),
+                       (out2[b * bo:b * bo + n], cs[2][b].astype(np.complex128))):
+            ref = O.direct_type3(xs, c, ss, sign)
+            alg = O.nufft3(xs, c, ss, sign, tol, bounds=b64)
+            got = got.astype(np.complex128)
+            assert O.rel_l2(got, ref) < 1.2 * O.rel_l2(alg, ref) + 2e-5
+            assert O.rel_l2(got, alg) < 5e-5
