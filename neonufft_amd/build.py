@@ -42,6 +42,7 @@ SOURCES = [
     ("grid.hip", True),
     ("pruned.hip", True),
     ("noalloc.hip", True),
+    ("rows.hip", True),
     ("params.cpp", False),
     ("fft.cpp", False),
     ("plan.cpp", False),

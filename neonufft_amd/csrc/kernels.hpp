@@ -239,6 +239,37 @@ void launch_band_spread(int ns, const BandJob<T>* jobs, const BandJob<T>& single
                         const EsParams& es, hipStream_t stream);
 std::size_t band_lds_bytes(int rows, int pitch, std::size_t acc_bytes);
 
+// ---- rows.hip: x-FFT in LDS joined to the tile reduction / zero padding ---
+// Pencils of `rows` (a power of two) consecutive y rows of one z plane with
+// the whole x extent n0 (a power of two >= 32); LDS rows of `pitch` cells
+// (n0 + 8). h: the pruned FFT's intermediate, h[(z N0 + x') n1 + y] for the
+// N0 retained x modes (retained order), not deconvolved.
+template <typename T>
+struct RowFftJob {
+  const Cplx<T>* tiles;       // type 1: spreading tiles, tile_cells(g) per subproblem
+  const uint32_t* sub_start;  // type 1: nbins + 1 subproblem offsets per bin
+  BinGeom g;                  // fine grid and its bins (type 1: bins_regular)
+  Cplx<T>* h;
+  Cplx<T>* grid;              // type 2: fine grid written (x fastest)
+  const Cplx<T>* tw;          // twiddles as for the row bands (tw[N/2 + q])
+  int N0, log2n0, rows, log2rows, pitch;
+};
+// Entries of a reduction pencil's subproblem-range table: every x bin x the y
+// bins its rows meet plus one on each side x three z bins (3D).
+inline int row_fft_rng_entries(const BinGeom& g, int rows, int dim) {
+  return g.nbins[0] * ((rows - 1) / g.bin[1] + 4) * (dim >= 3 ? 3 : 1);
+}
+inline std::size_t row_fft_lds_bytes(int rows, int pitch, std::size_t tbytes, int rng_entries) {
+  return 2 * static_cast<std::size_t>(rows) * static_cast<std::size_t>(pitch) * tbytes +
+         8 * static_cast<std::size_t>(rng_entries);
+}
+// Type 1: tiles -> h for every pencil of the grid (one workgroup each).
+template <typename T, int DIM>
+void launch_reduce_rows_fft(const RowFftJob<T>& job, int ns, hipStream_t stream);
+// Type 2: h -> fine grid.
+template <typename T>
+void launch_expand_rows_ifft(const RowFftJob<T>& job, hipStream_t stream);
+
 // ---- grid.hip: deconvolution factors, up/down-sampling, phases -----------
 template <typename T>
 void launch_fill_zero(Cplx<T>* p, std::size_t count, hipStream_t stream);

@@ -100,6 +100,21 @@ inline bool bands_suitable(const std::vector<uint32_t>& row_start, int n1, int r
   return true;
 }
 
+// Pencil height of the row kernels (rows.hip) for a 2D / 3D grid: the largest
+// power of two <= 16 whose LDS (two row planes of n0 + 8 cells plus the
+// subproblem-range table) fits kBandBudget; 0 when the row kernels do not
+// apply (n0 not a power of two >= 32, or not even one row fits).
+inline int row_fft_rows(int dim, const BinGeom& g, std::size_t tbytes) {
+  const int n0 = g.n[0];
+  if (dim < 2 || n0 < 32 || (n0 & (n0 - 1)) != 0) return 0;
+  for (int rows = 16; rows >= 1; rows /= 2) {
+    if (row_fft_lds_bytes(rows, n0 + 8, tbytes, row_fft_rng_entries(g, rows, dim)) <= kBandBudget) {
+      return rows;
+    }
+  }
+  return 0;
+}
+
 inline int log2_exact(IntType v) {
   int l = 0;
   while ((IntType(1) << l) < v) ++l;
@@ -166,8 +181,12 @@ public:
       band_ = plan_band_layout(n_, ks_.ns, sizeof(T), tune_, 256);
       band_rows_ = band_.rows;
       if (band_rows_ > 0) bounds_.reset(band_bound_words(static_cast<int>(n_[1]), sizeof(T)), alloc_);
-      if (band_.fft) upload_twiddles<T>(tw_, n_[0], sign_, stream_, alloc_);
     }
+    // row kernels: the x-FFT in LDS joined to the tile reduction (type 1) and
+    // to the zero padding (type 2); NNF_SPREAD_TILES keeps the separate
+    // reduction, rocFFT x pass and transposes
+    if (tune_.spread_mode != kSpreadTiles) row_rows_ = row_fft_rows(DIM, geom_, sizeof(T));
+    if ((DIM == 2 && band_.fft) || row_rows_ > 0) upload_twiddles<T>(tw_, n_[0], sign_, stream_, alloc_);
     set_points(num_nu, loc);
   }
 
@@ -325,6 +344,15 @@ public:
         return;
       }
     }
+    if constexpr (DIM >= 2) {
+      if (row_rows_ > 0 && !bands_on_ && bins_regular(DIM, geom_, ks_.ns)) {
+        // tiles -> retained x modes in one pass, then the remaining passes
+        spread_tiles(in);
+        launch_reduce_rows_fft<T, DIM>(row_job(), ks_.ns, stream_);
+        pfft_.forward_from_h(fs, order_fft(), out, nullptr, out_strides, stream_);
+        return;
+      }
+    }
     spread(in);
     if constexpr (DIM == 1) {
       fft_.execute(grid_.data());
@@ -343,7 +371,12 @@ public:
       launch_upsample<T, DIM>(order_fft(), in, in_strides, modes_, fs, grid_.data(), n_, stream_);
       fft_.execute(grid_.data());
     } else {
-      pfft_.backward(in, nullptr, in_strides, fs, order_fft(), grid_.data(), stream_);
+      if (row_rows_ > 0) {
+        pfft_.backward_to_h(in, nullptr, in_strides, fs, order_fft(), stream_);
+        launch_expand_rows_ifft<T>(row_job(), stream_);
+      } else {
+        pfft_.backward(in, nullptr, in_strides, fs, order_fft(), grid_.data(), stream_);
+      }
     }
     interp(out);
   }
@@ -379,6 +412,22 @@ private:
     if (!points_valid_) {
       throw neonufft::InputError("Neonufft:: no valid points (the last set_points failed)");
     }
+  }
+
+  RowFftJob<T> row_job() const {
+    RowFftJob<T> j{};
+    j.tiles = tiles_.data();
+    j.sub_start = pts_.sub_starts();
+    j.g = geom_;
+    j.h = pfft_.h();
+    j.grid = grid_.data();
+    j.tw = tw_.data();
+    j.N0 = static_cast<int>(modes_[0]);
+    j.log2n0 = log2_exact(n_[0]);
+    j.rows = row_rows_;
+    j.log2rows = log2_exact(row_rows_);
+    j.pitch = static_cast<int>(n_[0] + 8);
+    return j;
   }
 
   BandJob<T> band_job(const Cplx<T>* in) const {
@@ -420,6 +469,7 @@ private:
   IntType n_[3];
   BinGeom geom_{};
   int band_rows_ = 0;             // 2D row-band spreading (0 = subgrid tiles)
+  int row_rows_ = 0;              // pencil height of the row kernels (0 = not used)
   bool bands_on_ = false;         // row bands used for the current points
   bool points_valid_ = false;     // the last set_points succeeded
   std::vector<uint32_t> row_host_;  // row offsets of the view (balance check)

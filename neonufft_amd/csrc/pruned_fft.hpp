@@ -70,20 +70,41 @@ public:
     }
   }
 
-  // Intermediate of the 2D forward transform: per grid, N0 x-modes (retained
+  // Intermediate after the x pass: per grid and z plane, N0 x-modes (retained
   // order) of n1 cells each, y fastest. A caller that produces it directly
-  // (row-band spreading with its own x-FFT) continues with forward_from_h.
+  // (row-band spreading or the row reduction with their own x-FFT) continues
+  // with forward_from_h; type 2 stops there with backward_to_h.
   Cplx<T>* h() const { return h_.data(); }
 
-  // 2D type 1 from a filled h(): y-FFT and deconvolution into the outputs.
+  // Type 1 from a filled h() (x-FFT done, retained x modes written there by
+  // the row-band or row-reduction kernels): the remaining passes and the
+  // deconvolution into the outputs.
   void forward_from_h(T* const* fs, int order_fft, Cplx<T>* single, Cplx<T>* const* outs,
                       const IntType* st, hipStream_t s) {
-    static_assert(DIM == 2, "2D only");
-    const long long n1 = n_[1], N0 = N_[0], N1 = N_[1];
     py_.execute(h_.data());
-    launch_modes_out<T, 2>(h_.data(), N0 * n1, single, outs, count_, static_cast<int>(N0), 1,
-                           static_cast<int>(n1), static_cast<int>(N1), st[0], 0, st[1], fs[0],
-                           fs[1], fs[1], order_fft, s);
+    finish_forward(fs, order_fft, single, outs, st, s);
+  }
+
+  // Type 2 up to h(): deconvolution, zero padding and the passes after x; the
+  // caller pads and transforms x (row kernel) into the fine grid.
+  void backward_to_h(const Cplx<T>* single, const Cplx<T>* const* ins, const IntType* st,
+                     T* const* fs, int order_fft, hipStream_t s) {
+    const long long n1 = n_[1], n2 = n_[2], N0 = N_[0], N1 = N_[1], N2 = N_[2];
+    if constexpr (DIM == 2) {
+      launch_modes_in<T, 2>(single, ins, count_, h_.data(), N0 * n1, static_cast<int>(N0), 1,
+                            static_cast<int>(n1), static_cast<int>(N1), st[0], 0, st[1], fs[0],
+                            fs[1], fs[1], order_fft, s);
+    } else {
+      launch_modes_in<T, 3>(single, ins, count_, k_.data(), N1 * N0 * n2, static_cast<int>(N0),
+                            static_cast<int>(N1), static_cast<int>(n2), static_cast<int>(N2),
+                            st[0], st[1], st[2], fs[0], fs[1], fs[2], order_fft, s);
+      pz_.execute(k_.data());
+      launch_expand_transpose<T>(k_.data(), h_.data(), static_cast<int>(n2), static_cast<int>(n1),
+                                 static_cast<int>(N1), N0 * n2, N0 * n1,
+                                 static_cast<std::size_t>(count_ * N0), static_cast<int>(N0),
+                                 N1 * N0 * n2, n2, n2 * N0 * n1, n1, s);
+    }
+    py_.execute(h_.data());
   }
 
   void set_stream(hipStream_t s) {
@@ -96,7 +117,7 @@ public:
   // strides; outputs `single` (count = 1) or the device table `outs`.
   void forward(Cplx<T>* grids, T* const* fs, int order_fft, Cplx<T>* single,
                Cplx<T>* const* outs, const IntType* st, hipStream_t s) {
-    const long long n0 = n_[0], n1 = n_[1], n2 = n_[2], N0 = N_[0], N1 = N_[1], N2 = N_[2];
+    const long long n0 = n_[0], n1 = n_[1], n2 = n_[2], N0 = N_[0];
     const long long gstride = n0 * n1 * n2;
     px_.execute(grids);
     launch_compact_transpose<T>(grids, h_.data(), static_cast<int>(n1), static_cast<int>(n0),
@@ -104,6 +125,29 @@ public:
                                 static_cast<std::size_t>(count_ * n2), static_cast<int>(n2),
                                 gstride, n1 * n0, n2 * N0 * n1, N0 * n1, s);
     py_.execute(h_.data());
+    finish_forward(fs, order_fft, single, outs, st, s);
+  }
+
+  // Type 2: modes (caller's order and strides; `single` or the device table
+  // `ins`) -> deconvolved, zero-padded and transformed fine grids.
+  void backward(const Cplx<T>* single, const Cplx<T>* const* ins, const IntType* st,
+                T* const* fs, int order_fft, Cplx<T>* grids, hipStream_t s) {
+    const long long n0 = n_[0], n1 = n_[1], n2 = n_[2], N0 = N_[0];
+    const long long gstride = n0 * n1 * n2;
+    backward_to_h(single, ins, st, fs, order_fft, s);
+    launch_expand_transpose<T>(h_.data(), grids, static_cast<int>(n1), static_cast<int>(n0),
+                               static_cast<int>(N0), n1, n0,
+                               static_cast<std::size_t>(count_ * n2), static_cast<int>(n2),
+                               n2 * N0 * n1, N0 * n1, gstride, n1 * n0, s);
+    px_.execute(grids);
+  }
+
+private:
+  // After the y-FFT of h: 2D deconvolves from h; 3D transposes to K, runs the
+  // z-FFT and deconvolves.
+  void finish_forward(T* const* fs, int order_fft, Cplx<T>* single, Cplx<T>* const* outs,
+                      const IntType* st, hipStream_t s) {
+    const long long n1 = n_[1], n2 = n_[2], N0 = N_[0], N1 = N_[1], N2 = N_[2];
     if constexpr (DIM == 2) {
       launch_modes_out<T, 2>(h_.data(), N0 * n1, single, outs, count_, static_cast<int>(N0), 1,
                              static_cast<int>(n1), static_cast<int>(N1), st[0], 0, st[1], fs[0],
@@ -120,35 +164,6 @@ public:
     }
   }
 
-  // Type 2: modes (caller's order and strides; `single` or the device table
-  // `ins`) -> deconvolved, zero-padded and transformed fine grids.
-  void backward(const Cplx<T>* single, const Cplx<T>* const* ins, const IntType* st,
-                T* const* fs, int order_fft, Cplx<T>* grids, hipStream_t s) {
-    const long long n0 = n_[0], n1 = n_[1], n2 = n_[2], N0 = N_[0], N1 = N_[1], N2 = N_[2];
-    const long long gstride = n0 * n1 * n2;
-    if constexpr (DIM == 2) {
-      launch_modes_in<T, 2>(single, ins, count_, h_.data(), N0 * n1, static_cast<int>(N0), 1,
-                            static_cast<int>(n1), static_cast<int>(N1), st[0], 0, st[1], fs[0],
-                            fs[1], fs[1], order_fft, s);
-    } else {
-      launch_modes_in<T, 3>(single, ins, count_, k_.data(), N1 * N0 * n2, static_cast<int>(N0),
-                            static_cast<int>(N1), static_cast<int>(n2), static_cast<int>(N2),
-                            st[0], st[1], st[2], fs[0], fs[1], fs[2], order_fft, s);
-      pz_.execute(k_.data());
-      launch_expand_transpose<T>(k_.data(), h_.data(), static_cast<int>(n2), static_cast<int>(n1),
-                                 static_cast<int>(N1), N0 * n2, N0 * n1,
-                                 static_cast<std::size_t>(count_ * N0), static_cast<int>(N0),
-                                 N1 * N0 * n2, n2, n2 * N0 * n1, n1, s);
-    }
-    py_.execute(h_.data());
-    launch_expand_transpose<T>(h_.data(), grids, static_cast<int>(n1), static_cast<int>(n0),
-                               static_cast<int>(N0), n1, n0,
-                               static_cast<std::size_t>(count_ * n2), static_cast<int>(n2),
-                               n2 * N0 * n1, N0 * n1, gstride, n1 * n0, s);
-    px_.execute(grids);
-  }
-
-private:
   long long n_[3] = {1, 1, 1}, N_[3] = {1, 1, 1};
   int count_ = 0;
   FftPlan px_, py_, pz_;

@@ -381,3 +381,65 @@ def test_type1_clustered_rows_fall_back_to_tiles(prec):
     batch.transform_type_1([cin, cin[:500]], bo)
     torch.cuda.synchronize()
     assert O.rel_l2(bo[0].cpu().numpy().astype(np.complex128), outs["auto"]) < same
+
+
+ROW_CASES = [
+    # (dim, modes, M, tol, sign): fine grids with a power-of-two x extent, so
+    # type 1 joins the tile reduction to the x-FFT and type 2 the zero padding
+    # (x extent >= 2 bins, so the tile reduction is the bin-regular one)
+    (3, (32, 12, 20), 4000, 1e-5, -1),
+    (3, (64, 40, 24), 20000, 1e-6, 1),
+    (2, (128, 50), 40000, 1e-5, 1),
+    (2, (64, 200), 60000, 1e-4, -1),
+]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("case", ROW_CASES,
+                         ids=[f"d{c[0]}_m{'x'.join(map(str, c[1]))}_tol{c[3]:g}" for c in ROW_CASES])
+def test_row_fft_paths(case, prec):
+    """Type 1 / type 2 through the row kernels (x-FFT in LDS joined to the
+    tile reduction and to the zero padding) against the FP64 direct sum and
+    the FP64 restatement, and against the separate reduction + rocFFT x pass
+    (spread_mode="tiles") of the same points. 2D plans use subgrid tiles
+    for these points (clustered in a few rows), so type 1 takes the row
+    reduction there too."""
+    dim, modes, m, tol, sign = case
+    if prec == "f64":
+        tol *= 1e-3
+    rng = np.random.default_rng(int(np.prod(modes)) + m)
+    x = points(rng, dim, m, prec)
+    if dim == 2:
+        # all points within 3 rows of the 2D grid: the row bands refuse them
+        x[1] = (rng.uniform(-0.02, 0.02, m) + 1.0).astype(NPR[prec])
+    c = cplx(rng, m, prec)
+    f = cplx(rng, int(np.prod(modes)), prec)
+    loc = [dev(v, RDT[prec]) for v in x]
+    opt = nnf.Options(tol=tol)
+    fused = nnf.Plan(opt, sign, m, loc, modes)
+    sep = nnf.Plan(opt, sign, m, loc, modes, spread_mode="tiles")
+    assert fused.info()["spread_mode"] == "tiles"
+    nm = int(np.prod(modes))
+    outs = {}
+    for name, plan in (("fused", fused), ("sep", sep)):
+        f1 = torch.zeros(nm, dtype=CDT[prec], device="cuda")
+        plan.transform_type_1(dev(c, CDT[prec]), f1)
+        c2 = torch.zeros(m, dtype=CDT[prec], device="cuda")
+        plan.transform_type_2(dev(f, CDT[prec]), None, c2)
+        torch.cuda.synchronize()
+        outs[name] = (f1.cpu().numpy().astype(np.complex128), c2.cpu().numpy().astype(np.complex128))
+    xs = [v.astype(np.float64) for v in x]
+    amp = amplification(modes, tol, 2.0)
+    idx = sample_modes(modes, rng)
+    ref1 = O.direct_type1(xs, c.astype(np.complex128), O.mode_vectors(modes, idx), sign)
+    alg1 = O.nufft1(xs, c.astype(np.complex128), modes, sign, tol)
+    ref2 = O.direct_type2(xs, f.astype(np.complex128), modes, sign)
+    alg2 = O.nufft2(xs, f.astype(np.complex128), modes, sign, tol)
+    for name, (g1, g2) in outs.items():
+        assert O.rel_l2(g1[idx], ref1) < bound(prec, O.rel_l2(alg1[idx], ref1), tol, amp), name
+        assert O.rel_l2(g1, alg1) < alg_bound(prec, tol, amp), name
+        assert O.rel_l2(g2, ref2) < bound(prec, O.rel_l2(alg2, ref2), tol, amp), name
+        assert O.rel_l2(g2, alg2) < alg_bound(prec, tol, amp), name
+    same = 1e-12 if prec == "f64" else 2e-6
+    assert O.rel_l2(outs["fused"][0], outs["sep"][0]) < same
+    assert O.rel_l2(outs["fused"][1], outs["sep"][1]) < same
