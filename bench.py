@@ -711,10 +711,10 @@ def run_uniform(args):
     if prof:
         busy = {}
         for k in dom["kernels"]:
-            c = prof.get(k, {})
-            if "lds_busy_frac" in c:
-                busy[k] = {"lds_busy_frac": c["lds_busy_frac"],
-                           "bank_conflict_frac_of_busy": c.get("bank_conflict_frac_of_busy")}
+            ctr = prof.get(k, {})
+            if "lds_busy_frac" in ctr:
+                busy[k] = {"lds_busy_frac": ctr["lds_busy_frac"],
+                           "bank_conflict_frac_of_busy": ctr.get("bank_conflict_frac_of_busy")}
         lds["rocprof"] = busy
     bound = "lds" if lds_bps / LDS_PEAK > hbm_gbs / HBM_PEAK else "hbm"
 
