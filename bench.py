@@ -365,18 +365,20 @@ def cpu_leg_type2(x, f, out_gpu, modes, tol, sample, nacc):
     m = x[0].shape[0]
     xs = [v.astype(np.float64) for v in x]
     f64 = f.astype(np.complex128)
-    pidx = np.sort(rng.choice(m, size=min(nacc, m), replace=False))
+    ms = min(sample, m)
+    # accuracy samples among the points the timed restatement covers, so its
+    # output gives the same algorithm's FP64 error on the same samples
+    pidx = np.sort(rng.choice(ms, size=min(nacc, ms), replace=False))
     ref = O.direct_type2([v[pidx] for v in xs], f64, modes, -1)
     err = O.rel_l2(out_gpu[pidx], ref)
-    ms = min(sample, m)
     t0 = time.perf_counter()
-    O.nufft2([v[:ms] for v in xs], f64, modes, -1, tol)
+    alg = O.nufft2([v[:ms] for v in xs], f64, modes, -1, tol)
     t1 = time.perf_counter()
     cpu = {"value": ms / (t1 - t0), "unit": "NU pts/s", **host_cpu(),
            "kind": "port",
            "sample": (f"FP64 C restatement (oracle/, OpenMP) + numpy FFT, type 2 on the first "
                       f"{ms} points of the workload, same grid: {t1 - t0:.2f}s")}
-    return cpu, err
+    return cpu, err, O.rel_l2(alg[pidx], ref)
 
 
 def clustered_leg(args):
@@ -444,8 +446,9 @@ def clustered_leg(args):
     for r in results:
         o = r.pop("out")
         if o is not None:
-            c, err = cpu_leg_type2(x, f, o, modes, r["tol"], args.cpu_sample // 2, 200)
+            c, err, alg_err = cpu_leg_type2(x, f, o, modes, r["tol"], args.cpu_sample // 2, 200)
             r["rel_l2_vs_direct"] = err
+            r["fp64_restatement_rel_l2"] = alg_err
             r["accuracy_samples"] = 200
             if cpu is None:
                 cpu = c
@@ -545,6 +548,15 @@ def type3_leg(args):
             cc = c.astype(np.complex64 if prec == "f32" else np.complex128).astype(np.complex128)
             ref = O.direct_type3(xs, cc, ss, -1)
             res[prec]["rel_l2_vs_direct"] = O.rel_l2(outs[prec][tidx], ref)
+            if prec == "mixed":
+                # the same algorithm in FP64 on the same targets (grids sized
+                # from the full point sets' extents, as the plan's are)
+                bnd = ([float(v.min()) for v in x], [float(v.max()) for v in x],
+                       [float(v.min()) for v in s], [float(v.max()) for v in s])
+                alg = O.nufft3(xs, cc, ss, -1, 1e-6, bounds=bnd)
+                fp64_alg_err = O.rel_l2(alg, ref)
+                res["mixed"]["fp64_restatement_rel_l2"] = fp64_alg_err
+                res["f64"]["fp64_restatement_rel_l2"] = fp64_alg_err
         ms_, ns_ = args.cpu_sample // 20, args.cpu_sample // 40
         t0 = time.perf_counter()
         O.nufft3([v[:ms_] for v in x], c[:ms_], [v[:ns_] for v in s], -1, 1e-6)
