@@ -265,8 +265,12 @@ __global__ void __launch_bounds__(kBandThreads<T>)
   const int n0 = job.n0, n1 = job.n1, pitch = job.pitch;
   const int y0 = static_cast<int>(blockIdx.x - job.band_off) * job.rows;
   const int rk = min(job.rows, n1 - y0);
+  // Row r of the band: cell x in [-kBandLeft, pitch - kBandLeft) at
+  // r * pitch + kBandLeft + x of the real plane, `plane` cells later in the
+  // imaginary plane (the cells outside [0, n0) are the halo).
   Acc* tre = reinterpret_cast<Acc*>(smem_raw);
-  Acc* tim = tre + rk * pitch;
+  const int plane = rk * pitch;
+  Acc* tim = tre + plane;
   for (int i = threadIdx.x; i < 2 * rk * pitch; i += NT) tre[i] = Acc(0);
 
   // Points whose first kernel row lies in [y0 - NS + 1, y0 + rk - 1] (mod
@@ -356,12 +360,10 @@ __global__ void __launch_bounds__(kBandThreads<T>)
       const int l0 = valid ? static_cast<int>(rec[k].cell >> CellPack<2>::bits) + lr0[k] : rk;
       T kx[NS];
       es_weights<T, NS>(rec[k].x[0], es, kx);
-      int xs[NS];
-#pragma unroll
-      for (int a = 0; a < NS; ++a) {
-        xs[a] = wrap_once(x0 + a, n0);
-
-      }
+      // cells x0 .. x0 + NS - 1 of the row, unwrapped: a run that crosses
+      // either end of the row lands in the row's halo and is folded below,
+      // so the NS atomics of a row share one address register
+      const int xoff = kBandLeft + x0;
 #pragma unroll
       for (int b = 0; b < NS; ++b) {
         const int r = l0 + b;
@@ -370,28 +372,41 @@ __global__ void __launch_bounds__(kBandThreads<T>)
         if (act) {
           const T wy = es_piece<NS>(rec[k].x[1], b, es);
           const T vr = wy * cr, vi = wy * ci;
-          Acc* rr = tre + r * pitch;
-          Acc* ri = tim + r * pitch;
+          Acc* rr = tre + r * pitch + xoff;
+          Acc* ri = rr + plane;
 #pragma unroll
           for (int a = 0; a < NS; ++a) {
-            lds_add_fixed(&rr[xs[a]], to_fixed(kx[a] * vr));
-            lds_add_fixed(&ri[xs[a]], to_fixed(kx[a] * vi));
+            lds_add_fixed(&rr[a], to_fixed(kx[a] * vr));
+            lds_add_fixed(&ri[a], to_fixed(kx[a] * vi));
           }
         }
       }
     }
   }
   __syncthreads();
+  // periodic fold of the halos: x in [-NS/2, 0) belongs to x + n0, x in
+  // [n0, n0 + NS - NS/2) to x - n0 (sources and targets are disjoint)
+  for (int i = threadIdx.x; i < rk * NS; i += NT) {
+    const int r = i / NS;
+    const int j = i - r * NS;
+    constexpr int H = NS / 2;
+    const int src = j < H ? j - H : n0 + (j - H);
+    const int dst = j < H ? src + n0 : src - n0;
+    Acc* row = tre + r * pitch + kBandLeft;
+    row[dst] += row[src];
+    row[plane + dst] += row[plane + src];
+  }
+  __syncthreads();
 
   const T inv = bound > T(0) ? bound / one : T(0);
   if constexpr (FFT) {
-    lds_fft_x<T, NT, true>(tre, tim, rk, pitch, job.log2n0, inv, job.tw);
+    lds_fft_x<T, NT, true>(tre + kBandLeft, tim + kBandLeft, rk, pitch, job.log2n0, inv, job.tw);
     // retained mode xp of row r: X[f] at position bitrev(f). Lanes take
     // (r fastest, then xp from eight interleaved strides of g modes) so that
     // eight lanes write 8 rows of one mode contiguously and the bitrev
     // positions of a wave spread over the LDS banks.
-    const T* fre = reinterpret_cast<const T*>(tre);
-    const T* fim = reinterpret_cast<const T*>(tim);
+    const T* fre = reinterpret_cast<const T*>(tre + kBandLeft);
+    const T* fim = reinterpret_cast<const T*>(tim + kBandLeft);
     const int lr = job.log2rows;
     const int N0 = job.N0;
     const int P = N0 - N0 / 2;
@@ -421,8 +436,8 @@ __global__ void __launch_bounds__(kBandThreads<T>)
     for (int r = tr; r < rk; r += rpp) {
       Cplx<T>* row = grid + static_cast<std::size_t>(r) * n0;
       for (int x = tx; x < n0; x += (n0 >= NT ? NT : n0)) {
-        Cplx<T> v{static_cast<T>(tre[r * pitch + x]) * inv,
-                  static_cast<T>(tim[r * pitch + x]) * inv};
+        Cplx<T> v{static_cast<T>(tre[r * pitch + kBandLeft + x]) * inv,
+                  static_cast<T>(tim[r * pitch + kBandLeft + x]) * inv};
         if (accumulate) {
           const Cplx<T> o = row[x];
           v.x += o.x;

@@ -238,6 +238,12 @@ void launch_band_spread(int ns, const BandJob<T>* jobs, const BandJob<T>& single
                         std::size_t total_bands, std::size_t lds_bytes, bool accumulate, bool fft,
                         const EsParams& es, hipStream_t stream);
 std::size_t band_lds_bytes(int rows, int pitch, std::size_t acc_bytes);
+// LDS row of a band: cells x in [-kBandLeft, pitch - kBandLeft) (the cells
+// outside [0, n0) catch kernel runs that cross the row's ends and are folded
+// back periodically); pitch = n0 + kBandPad keeps every row's cell 0 32-byte
+// aligned for the in-LDS FFT and shifts consecutive rows by 24 banks.
+constexpr int kBandLeft = 8;
+constexpr int kBandPad = 24;
 
 // ---- rows.hip: x-FFT in LDS joined to the tile reduction / zero padding ---
 // Pencils of `rows` (a power of two) consecutive y rows of one z plane with

@@ -37,9 +37,10 @@ struct PlanTuning {
 
 // Row-band layout of a 2D plan: rows = 0 when subgrid tiles are used (asked
 // for, or the grid does not suit the row view). fft: type 1 runs the x-FFT
-// inside the band kernel (n0 a power of two >= 32, rows a power of two; LDS
-// rows padded by 8 words so that the transposed mode write-out spreads over
-// the banks).
+// inside the band kernel (n0 a power of two >= 32, rows a power of two). LDS
+// rows have pitch n0 + kBandPad (halo cells for runs crossing the row ends,
+// and a shift of the banks from row to row for the transposed mode
+// write-out).
 struct BandLayout {
   int rows = 0;
   int pitch = 0;
@@ -53,7 +54,7 @@ inline BandLayout plan_band_layout(const IntType* n, int ns, std::size_t acc_byt
   const IntType n0 = n[0];
   const bool pow2 = n0 >= 32 && (n0 & (n0 - 1)) == 0;
   b.fft = pow2 && t.spread_mode != kSpreadBandsGrid;
-  b.pitch = static_cast<int>(b.fft ? n0 + 8 : n0);
+  b.pitch = static_cast<int>(n0 + kBandPad);
   int rows = band_rows(n0, b.pitch, n[1], ns, acc_bytes, kBandBudget, min_bands);
   if (rows > 0 && t.band_rows > 0) {
     rows = std::min(band_rows(n0, b.pitch, n[1], ns, acc_bytes, kBandBudget, 1), t.band_rows);

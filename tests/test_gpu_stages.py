@@ -155,9 +155,9 @@ def test_band_spread(case, prec):
     if rows == -1:
         rows = n1 - ns + 1
     # requested rows, capped by the 68 KiB LDS band (two int32 / int64
-    # accumulator planes per row of n0 words, n0 + 8 with the fused x-FFT)
-    # and rounded down to a power of two with the fused x-FFT
-    pitch = n0 + 8 if fused else n0
+    # accumulator planes per row of n0 + 24 words: halo cells and a bank
+    # shift) and rounded down to a power of two with the fused x-FFT
+    pitch = n0 + 24
     expect = min(rows, n1 - ns + 1, (68 * 1024) // (2 * pitch * (4 if prec == "f32" else 8)))
     if fused and expect > 0:
         expect = 1 << (expect.bit_length() - 1)
