@@ -79,18 +79,16 @@ __global__ void __launch_bounds__(kRowThreads<T>)
   }
   __syncthreads();
 
-  // Per dimension of a cell: own bin (as its entry index along that
-  // dimension of the range table) and position in its tile, and the one
+  // Cover of a coordinate along dimension d: own bin (as its entry index
+  // along d in the range table) and position in that bin's tile, and the one
   // neighbour whose halo also covers it (entry -1 if none) with its position.
   struct Cover {
     int jown, pown, joth, poth;
   };
-  const float rb0 = 1.0f / static_cast<float>(g.bin[0]);
-  const float rb1 = 1.0f / static_cast<float>(g.bin[1]);
-  auto cover = [&](int c, int d, float rb, int lo) {
+  auto cover = [&](int c, int d, int lo) {
     Cover cv;
     int bd;
-    const int li = div_rem(c, g.bin[d], rb, &bd);
+    const int li = div_rem(c, g.bin[d], 1.0f / static_cast<float>(g.bin[d]), &bd);
     const int w = min(g.bin[d], g.n[d] - bd * g.bin[d]);
     const int nb = g.nbins[d];
     // entry of bin b along d: (b - lo) mod nb (lo = first bin of the table)
@@ -114,64 +112,73 @@ __global__ void __launch_bounds__(kRowThreads<T>)
     }
     return cv;
   };
-  const std::size_t tc = tile_cells(g);
+  // y cover of every row of the pencil, z cover of its plane
+  __shared__ int ycov[16][4];
+  if (static_cast<int>(threadIdx.x) < pc.rk) {
+    const Cover cy = cover(pc.y0 + static_cast<int>(threadIdx.x), 1, by_lo);
+    ycov[threadIdx.x][0] = cy.jown;
+    ycov[threadIdx.x][1] = cy.pown;
+    ycov[threadIdx.x][2] = cy.joth;
+    ycov[threadIdx.x][3] = cy.poth;
+  }
   Cover cz{0, 0, -1, 0};
-  if constexpr (DIM >= 3) cz = cover(pc.z, 2, 1.0f / static_cast<float>(g.bin[2]), bz_lo);
+  if constexpr (DIM >= 3) cz = cover(pc.z, 2, bz_lo);
+  for (int i = threadIdx.x; i < 2 * job.rows * job.pitch; i += NT) fre[i] = T(0);
+  __syncthreads();
 
-  // Threads take x fastest; with n0 <= NT several rows run side by side and
-  // each thread keeps one x for the whole pencil.
-  const int xthreads = min(NT, n0);
-  const int rpar = NT / xthreads;
-  const int tx = static_cast<int>(threadIdx.x) % xthreads;
-  const int tr = static_cast<int>(threadIdx.x) / xthreads;
-  const Cover cx0 = cover(tx, 0, rb0, 0);
-  for (int r = tr; r < pc.rk; r += rpar) {
-    const Cover cy = cover(pc.y0 + r, 1, rb1, by_lo);
-    for (int x = tx; x < n0; x += xthreads) {
-      const Cover cx = x == tx ? cx0 : cover(x, 0, rb0, 0);
-      constexpr int kCombos = DIM >= 3 ? 8 : 4;
-      Cplx<T> v[kCombos];
-      int idx[kCombos];
-      std::size_t tpos[kCombos];
-#pragma unroll
-      for (int k = 0; k < kCombos; ++k) {
-        const bool ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-        const int jx = ox ? cx.joth : cx.jown;
-        const int jy = oy ? cy.joth : cy.jown;
-        const int jz = oz ? cz.joth : cz.jown;
-        idx[k] = (jx < 0 || jy < 0 || jz < 0) ? -1 : jx + nb0 * (jy + ny * jz);
-        const int p0 = ox ? cx.poth : cx.pown;
-        const int p1 = oy ? cy.poth : cy.pown;
-        const int p2 = DIM >= 3 ? (oz ? cz.poth : cz.pown) : 0;
-        tpos[k] = static_cast<std::size_t>(p0 + g.tile[0] * (p1 + g.tile[1] * p2));
-      }
-      // first subproblem of every covering bin: independent loads
-#pragma unroll
-      for (int k = 0; k < kCombos; ++k) {
-        v[k] = Cplx<T>{T(0), T(0)};
-        if (idx[k] >= 0 && rng[2 * idx[k] + 1] > rng[2 * idx[k]]) {
-          v[k] = job.tiles[tpos[k] + static_cast<std::size_t>(rng[2 * idx[k]]) * tc];
-        }
-      }
+  // Tile rows into the pencil, one x bin per item: the row segment of x bin
+  // bx's tiles (every subproblem, own and covering-neighbour y / z tiles)
+  // is summed and added to grid cells bx bin0 - ns/2 + j (mod n0),
+  // j < bin width + ns. Consecutive lanes take consecutive j, so tile rows
+  // are read coalesced. Segments of bins of one parity do not overlap, so
+  // the bins are added in two passes (three with an odd bin count: the last
+  // bin, whose wrapped segment meets bin 0's, alone).
+  const std::size_t tc = tile_cells(g);
+  const int t0 = g.tile[0], t1 = g.tile[1];
+  const int bin0 = g.bin[0];
+  const float rt0 = 1.0f / static_cast<float>(t0);
+  const int nphase = (nb0 & 1) ? 3 : 2;
+  for (int phase = 0; phase < nphase; ++phase) {
+    // bins of this phase: bx = phase, phase + 2, ... (the odd last bin: phase 2)
+    const int last = (nb0 & 1) ? nb0 - 1 : nb0;
+    const int nbp = phase == 2 ? 1 : (last - phase + 1) / 2;
+    const int items = pc.rk * nbp * t0;
+    for (int i = threadIdx.x; i < items; i += NT) {
+      int rest;
+      const int j = div_rem(i, t0, rt0, &rest);
+      const int r = rest / nbp;
+      const int bx = phase == 2 ? nb0 - 1 : phase + 2 * (rest - r * nbp);
+      const int ext = min(bin0, n0 - bx * bin0) + ns;
+      if (j >= ext) continue;
       T ar = T(0), ai = T(0);
 #pragma unroll
-      for (int k = 0; k < kCombos; ++k) {
-        ar += v[k].x;
-        ai += v[k].y;
-        if (idx[k] >= 0) {  // bins split into several subproblems
-          const uint32_t s1 = rng[2 * idx[k] + 1];
-          for (uint32_t s = rng[2 * idx[k]] + 1; s < s1; ++s) {
-            const Cplx<T> w = job.tiles[tpos[k] + static_cast<std::size_t>(s) * tc];
-            ar += w.x;
-            ai += w.y;
+      for (int oy = 0; oy < 2; ++oy) {
+        const int jy = ycov[r][2 * oy];
+        if (jy < 0) continue;
+        const int py = ycov[r][2 * oy + 1];
+#pragma unroll
+        for (int oz = 0; oz < (DIM >= 3 ? 2 : 1); ++oz) {
+          const int jz = oz ? cz.joth : cz.jown;
+          if (jz < 0) continue;
+          const int pz = oz ? cz.poth : cz.pown;
+          const int e = bx + nb0 * (jy + ny * jz);
+          const Cplx<T>* t =
+              job.tiles + static_cast<std::size_t>(j + t0 * (py + t1 * pz));
+          const uint32_t s1 = rng[2 * e + 1];
+          for (uint32_t sp = rng[2 * e]; sp < s1; ++sp) {
+            const Cplx<T> v = t[static_cast<std::size_t>(sp) * tc];
+            ar += v.x;
+            ai += v.y;
           }
         }
       }
-      fre[r * job.pitch + x] = ar;
-      fim[r * job.pitch + x] = ai;
+      int x = bx * bin0 - h + j;
+      x = x < 0 ? x + n0 : (x >= n0 ? x - n0 : x);
+      fre[r * job.pitch + x] += ar;
+      fim[r * job.pitch + x] += ai;
     }
+    __syncthreads();
   }
-  __syncthreads();
 
   lds_fft_x<T, NT, false>(fre, fim, pc.rk, job.pitch, job.log2n0, T(1), job.tw);
 
