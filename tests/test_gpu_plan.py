@@ -443,3 +443,44 @@ def test_row_fft_paths(case, prec):
     same = 1e-12 if prec == "f64" else 2e-6
     assert O.rel_l2(outs["fused"][0], outs["sep"][0]) < same
     assert O.rel_l2(outs["fused"][1], outs["sep"][1]) < same
+
+
+@pytest.mark.parametrize("dim,prec", [(3, "f32"), (2, "f64"), (2, "f32")])
+def test_row_fft_odd_bin_count(dim, prec):
+    """Row reduction over an odd number of x bins (three passes: the last
+    bin's wrapped tile row meets bin 0's) and a partial last bin, against the
+    FP64 direct sum and the separate reduction. (2D: points in a few rows, so
+    the plan uses subgrid tiles; fp64 3D tiles of this width exceed the LDS
+    budget.)"""
+    tol = 1e-5 if prec == "f32" else 1e-8
+    modes = (64, 20, 12) if dim == 3 else (64, 90)
+    m, sign = (6000 if dim == 3 else 60000), 1
+    rng = np.random.default_rng(505 + dim)
+    x = points(rng, dim, m, prec)
+    if dim == 2:
+        x[1] = (rng.uniform(-0.02, 0.02, m) - 2.0).astype(NPR[prec])
+    c = cplx(rng, m, prec)
+    loc = [dev(v, RDT[prec]) for v in x]
+    opt = nnf.Options(tol=tol)
+    bins = [48, 8, 8] if dim == 3 else [48, 32]
+    # 128 fine cells in x cut into 48-wide bins: 3 bins, the last 32 wide
+    fused = nnf.Plan(opt, sign, m, loc, modes, bin_size=bins)
+    sep = nnf.Plan(opt, sign, m, loc, modes, bin_size=bins, spread_mode="tiles")
+    info = fused.info()
+    assert info["grid"][0] == 128 and info["bin_size"][0] == 48, info
+    assert info["spread_mode"] == "tiles"
+    nm = int(np.prod(modes))
+    outs = []
+    for plan in (fused, sep):
+        f1 = torch.zeros(nm, dtype=CDT[prec], device="cuda")
+        plan.transform_type_1(dev(c, CDT[prec]), f1)
+        torch.cuda.synchronize()
+        outs.append(f1.cpu().numpy().astype(np.complex128))
+    xs = [v.astype(np.float64) for v in x]
+    idx = sample_modes(modes, rng)
+    ref = O.direct_type1(xs, c.astype(np.complex128), O.mode_vectors(modes, idx), sign)
+    alg = O.nufft1(xs, c.astype(np.complex128), modes, sign, tol)
+    amp = amplification(modes, tol, 2.0)
+    assert O.rel_l2(outs[0][idx], ref) < bound(prec, O.rel_l2(alg[idx], ref), tol, amp)
+    assert O.rel_l2(outs[0], alg) < alg_bound(prec, tol, amp)
+    assert O.rel_l2(outs[0], outs[1]) < (1e-12 if prec == "f64" else 2e-6)
