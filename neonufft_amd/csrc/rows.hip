@@ -112,17 +112,26 @@ __global__ void __launch_bounds__(kRowThreads<T>)
     }
     return cv;
   };
-  // y cover of every row of the pencil, z cover of its plane
-  __shared__ int ycov[16][4];
-  if (static_cast<int>(threadIdx.x) < pc.rk) {
-    const Cover cy = cover(pc.y0 + static_cast<int>(threadIdx.x), 1, by_lo);
-    ycov[threadIdx.x][0] = cy.jown;
-    ycov[threadIdx.x][1] = cy.pown;
-    ycov[threadIdx.x][2] = cy.joth;
-    ycov[threadIdx.x][3] = cy.poth;
+  // Per row of the pencil and per covering tile pair (own / neighbour y bin
+  // x own / neighbour z bin): the offset of the row's tile row inside a tile
+  // and the range-table entry of x bin 0 (the x bin adds to it), or -1 when
+  // that neighbour does not cover the row.
+  constexpr int NQ = DIM >= 3 ? 4 : 2;
+  __shared__ int combo[16][NQ][2];
+  if (static_cast<int>(threadIdx.x) < pc.rk * NQ) {
+    const int r = static_cast<int>(threadIdx.x) / NQ;
+    const int q = static_cast<int>(threadIdx.x) - r * NQ;
+    const int oy = DIM >= 3 ? q >> 1 : q;
+    const int oz = DIM >= 3 ? q & 1 : 0;
+    const Cover cy = cover(pc.y0 + r, 1, by_lo);
+    Cover cz{0, 0, -1, 0};
+    if constexpr (DIM >= 3) cz = cover(pc.z, 2, bz_lo);
+    const int jy = oy ? cy.joth : cy.jown, py = oy ? cy.poth : cy.pown;
+    const int jz = oz ? cz.joth : cz.jown, pz = oz ? cz.poth : cz.pown;
+    const bool ok = jy >= 0 && jz >= 0;
+    combo[r][q][0] = ok ? g.tile[0] * (py + g.tile[1] * pz) : -1;
+    combo[r][q][1] = ok ? nb0 * (jy + ny * jz) : 0;
   }
-  Cover cz{0, 0, -1, 0};
-  if constexpr (DIM >= 3) cz = cover(pc.z, 2, bz_lo);
   for (int i = threadIdx.x; i < 2 * job.rows * job.pitch; i += NT) fre[i] = T(0);
   __syncthreads();
 
@@ -130,11 +139,12 @@ __global__ void __launch_bounds__(kRowThreads<T>)
   // bx's tiles (every subproblem, own and covering-neighbour y / z tiles)
   // is summed and added to grid cells bx bin0 - ns/2 + j (mod n0),
   // j < bin width + ns. Consecutive lanes take consecutive j, so tile rows
-  // are read coalesced. Segments of bins of one parity do not overlap, so
-  // the bins are added in two passes (three with an odd bin count: the last
-  // bin, whose wrapped segment meets bin 0's, alone).
+  // are read coalesced; the first subproblem of every covering tile is
+  // loaded before any is waited for. Segments of bins of one parity do not
+  // overlap, so the bins are added in two passes (three with an odd bin
+  // count: the last bin, whose wrapped segment meets bin 0's, alone).
   const std::size_t tc = tile_cells(g);
-  const int t0 = g.tile[0], t1 = g.tile[1];
+  const int t0 = g.tile[0];
   const int bin0 = g.bin[0];
   const float rt0 = 1.0f / static_cast<float>(t0);
   const int nphase = (nb0 & 1) ? 3 : 2;
@@ -142,40 +152,73 @@ __global__ void __launch_bounds__(kRowThreads<T>)
     // bins of this phase: bx = phase, phase + 2, ... (the odd last bin: phase 2)
     const int last = (nb0 & 1) ? nb0 - 1 : nb0;
     const int nbp = phase == 2 ? 1 : (last - phase + 1) / 2;
+    const float rnbp = 1.0f / static_cast<float>(nbp);
     const int items = pc.rk * nbp * t0;
-    for (int i = threadIdx.x; i < items; i += NT) {
-      int rest;
-      const int j = div_rem(i, t0, rt0, &rest);
-      const int r = rest / nbp;
-      const int bx = phase == 2 ? nb0 - 1 : phase + 2 * (rest - r * nbp);
-      const int ext = min(bin0, n0 - bx * bin0) + ns;
-      if (j >= ext) continue;
-      T ar = T(0), ai = T(0);
+    // two items per lane per step, so that every tile load of both is in
+    // flight before the first wait
+    constexpr int U = 2;
+    for (int i0 = threadIdx.x; i0 < items; i0 += U * NT) {
+      int r[U], x[U];
+      bool live[U];
+      std::size_t o[U][NQ];
+      uint32_t s0[U][NQ], s1[U][NQ];
 #pragma unroll
-      for (int oy = 0; oy < 2; ++oy) {
-        const int jy = ycov[r][2 * oy];
-        if (jy < 0) continue;
-        const int py = ycov[r][2 * oy + 1];
+      for (int u = 0; u < U; ++u) {
+        const int i = min(i0 + u * NT, items - 1);
+        int rest;
+        const int j = div_rem(i, t0, rt0, &rest);
+        const int bxi = div_rem(rest, nbp, rnbp, &r[u]);
+        const int bx = phase == 2 ? nb0 - 1 : phase + 2 * bxi;
+        const int ext = min(bin0, n0 - bx * bin0) + ns;
+        live[u] = i0 + u * NT < items && j < ext;
+        int xx = bx * bin0 - h + j;
+        x[u] = xx < 0 ? xx + n0 : (xx >= n0 ? xx - n0 : xx);
+        // element offsets of the first subproblem's tile row segment; a
+        // pair that does not cover the row (its entry base is 0, so the
+        // range read stays in the table), a bin without subproblems or a
+        // dead item reads element 0 (the tile buffer always holds at least
+        // one tile) and is then dropped
 #pragma unroll
-        for (int oz = 0; oz < (DIM >= 3 ? 2 : 1); ++oz) {
-          const int jz = oz ? cz.joth : cz.jown;
-          if (jz < 0) continue;
-          const int pz = oz ? cz.poth : cz.pown;
-          const int e = bx + nb0 * (jy + ny * jz);
-          const Cplx<T>* t =
-              job.tiles + static_cast<std::size_t>(j + t0 * (py + t1 * pz));
-          const uint32_t s1 = rng[2 * e + 1];
-          for (uint32_t sp = rng[2 * e]; sp < s1; ++sp) {
-            const Cplx<T> v = t[static_cast<std::size_t>(sp) * tc];
-            ar += v.x;
-            ai += v.y;
-          }
+        for (int q = 0; q < NQ; ++q) {
+          const int off = combo[r[u]][q][0];
+          const int e = bx + combo[r[u]][q][1];
+          const uint32_t lo = rng[2 * e], hi = rng[2 * e + 1];
+          s0[u][q] = lo;
+          s1[u][q] = off >= 0 && live[u] ? hi : lo;
+          o[u][q] = lo < s1[u][q]
+                        ? static_cast<std::size_t>(lo) * tc + static_cast<std::size_t>(off + j)
+                        : std::size_t{0};
         }
       }
-      int x = bx * bin0 - h + j;
-      x = x < 0 ? x + n0 : (x >= n0 ? x - n0 : x);
-      fre[r * job.pitch + x] += ar;
-      fim[r * job.pitch + x] += ai;
+      const Cplx<T>* __restrict__ tiles = job.tiles;
+      Cplx<T> v[U][NQ];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) v[u][q] = tiles[o[u][q]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        T ar = T(0), ai = T(0);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const bool has = s0[u][q] < s1[u][q];
+          ar += has ? v[u][q].x : T(0);
+          ai += has ? v[u][q].y : T(0);
+        }
+        // further subproblems of bins split by their point count
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          for (uint32_t sp = s0[u][q] + 1; sp < s1[u][q]; ++sp) {
+            const Cplx<T> w = tiles[o[u][q] + static_cast<std::size_t>(sp - s0[u][q]) * tc];
+            ar += w.x;
+            ai += w.y;
+          }
+        }
+        if (live[u]) {
+          fre[r[u] * job.pitch + x[u]] += ar;
+          fim[r[u] * job.pitch + x[u]] += ai;
+        }
+      }
     }
     __syncthreads();
   }
