@@ -86,6 +86,14 @@ __device__ inline int unpack_cell(uint32_t cell, int d) {
 // plans) range less a margin for the rounding of each added term (at most
 // one half per add; a cell receives at most a few thousand adds). Finished
 // tiles are converted back once per cell.
+//
+// fp32 tiles pack a cell into one 64-bit word, re + 2^32 im, so that one
+// ds_add_u64 replaces two ds_add_u32 (the atomic rate, not the VALU, bounds
+// spreading). Each added word is the rounded (re_i, im_i) with the borrow of
+// a negative low word taken from the high one (high = im_i + (re_i >> 31)),
+// so the 64-bit sum is exactly sum re + 2^32 sum im; |sum re| < 2^31 gives
+// re = low word (signed) and im = high word + [re < 0]. The tile holds the
+// same integers as separate re / im planes would.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct FixedAcc;
@@ -114,6 +122,19 @@ __device__ inline void lds_add_fixed(int* p, int v) {
 }
 __device__ inline void lds_add_fixed(long long* p, long long v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// fp32 packed cell: adds re + 2^32 im (low word re, high word im plus the
+// borrow of a negative re).
+__device__ inline void lds_add_packed(unsigned long long* p, int re, int im) {
+  const unsigned long long w =
+      (static_cast<unsigned long long>(static_cast<uint32_t>(im + (re >> 31))) << 32) |
+      static_cast<uint32_t>(re);
+  __hip_atomic_fetch_add(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// Sum of packed words -> (re, im) in fixed point.
+__device__ inline void unpack_packed(unsigned long long w, int* re, int* im) {
+  *re = static_cast<int>(static_cast<uint32_t>(w));
+  *im = static_cast<int>(static_cast<uint32_t>(w >> 32)) + (*re < 0 ? 1 : 0);
 }
 
 }  // namespace

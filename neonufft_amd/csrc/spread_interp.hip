@@ -87,10 +87,13 @@ __device__ __forceinline__ void spread_subproblem(const SpreadJob<T, DIM>& job,
   const BinGeom& g = job.g;
   const TileInfo<DIM, NS> ti(sp.bin, g);
   const int tile_cells = g.tile[0] * (DIM >= 2 ? g.tile[1] : 1) * (DIM >= 3 ? g.tile[2] : 1);
-  // Real and imaginary parts live in separate planes so that 32-bit atomics
-  // of neighbouring cells fall in neighbouring banks.
+  // fp32: one packed 64-bit word (re + 2^32 im) per cell, one atomic per
+  // cell; fp64: real and imaginary parts in separate planes of 64-bit
+  // integers. Either way neighbouring cells fall in neighbouring banks.
+  constexpr bool kPacked = std::is_same_v<T, float>;
   Acc* tre = reinterpret_cast<Acc*>(smem_raw);
   Acc* tim = tre + tile_cells;
+  unsigned long long* tpk = reinterpret_cast<unsigned long long*>(smem_raw);
   for (int i = threadIdx.x; i < 2 * tile_cells; i += kThreads) tre[i] = Acc(0);
 
   // This thread's points (at most kPointsPerThread) and the tile bound.
@@ -128,8 +131,12 @@ __device__ __forceinline__ void spread_subproblem(const SpreadJob<T, DIM>& job,
     if constexpr (DIM == 1) {
 #pragma unroll
       for (int a = 0; a < NS; ++a) {
-        lds_add_fixed(&tre[r0 + a], to_fixed(kx[a] * c.x));
-        lds_add_fixed(&tim[r0 + a], to_fixed(kx[a] * c.y));
+        if constexpr (kPacked) {
+          lds_add_packed(&tpk[r0 + a], to_fixed(kx[a] * c.x), to_fixed(kx[a] * c.y));
+        } else {
+          lds_add_fixed(&tre[r0 + a], to_fixed(kx[a] * c.x));
+          lds_add_fixed(&tim[r0 + a], to_fixed(kx[a] * c.y));
+        }
       }
     } else {
       T ky[NS];
@@ -148,10 +155,22 @@ __device__ __forceinline__ void spread_subproblem(const SpreadJob<T, DIM>& job,
         for (int b = 0; b < NS; ++b) {
           const T vr = ky[b] * czr, vi = ky[b] * czi;
           const int base = ((r2 + cz) * ti.t1 + r1 + b) * ti.t0 + r0;
+          if constexpr (kPacked) {
+            // the row's conversions first: a conversion's result is not
+            // read by the next instruction (which would need a wait state)
+            int fr[NS], fi[NS];
 #pragma unroll
-          for (int a = 0; a < NS; ++a) {
-            lds_add_fixed(&tre[base + a], to_fixed(kx[a] * vr));
-            lds_add_fixed(&tim[base + a], to_fixed(kx[a] * vi));
+            for (int a = 0; a < NS; ++a) fr[a] = to_fixed(kx[a] * vr);
+#pragma unroll
+            for (int a = 0; a < NS; ++a) fi[a] = to_fixed(kx[a] * vi);
+#pragma unroll
+            for (int a = 0; a < NS; ++a) lds_add_packed(&tpk[base + a], fr[a], fi[a]);
+          } else {
+#pragma unroll
+            for (int a = 0; a < NS; ++a) {
+              lds_add_fixed(&tre[base + a], to_fixed(kx[a] * vr));
+              lds_add_fixed(&tim[base + a], to_fixed(kx[a] * vi));
+            }
           }
         }
       }
@@ -166,7 +185,13 @@ __device__ __forceinline__ void spread_subproblem(const SpreadJob<T, DIM>& job,
   Cplx<T>* __restrict__ dst =
       job.tiles + static_cast<std::size_t>(sidx - job.sub_off) * static_cast<std::size_t>(tile_cells);
   for (int i = threadIdx.x; i < tile_cells; i += kThreads) {
-    dst[i] = Cplx<T>{static_cast<T>(tre[i]) * inv, static_cast<T>(tim[i]) * inv};
+    if constexpr (kPacked) {
+      int re, im;
+      unpack_packed(tpk[i], &re, &im);
+      dst[i] = Cplx<T>{static_cast<T>(re) * inv, static_cast<T>(im) * inv};
+    } else {
+      dst[i] = Cplx<T>{static_cast<T>(tre[i]) * inv, static_cast<T>(tim[i]) * inv};
+    }
   }
 }
 
