@@ -29,7 +29,7 @@ __device__ inline int wrap_once(int i, int n) { return i < 0 ? i + n : (i >= n ?
 // q - 1, fixed by one correction. Returns r, sets *q.
 __device__ inline int div_rem(int i, int d, float rd, int* q) {
   int t = static_cast<int>(static_cast<float>(i) * rd);
-  int r = i - t * d;
+  int r = i - __mul24(t, d);  // t d <= i < 2^22: exact in the 24-bit multiplier
   if (r >= d) {
     ++t;
     r -= d;

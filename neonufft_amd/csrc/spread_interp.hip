@@ -244,6 +244,11 @@ __device__ __forceinline__ void interp_subproblem(const InterpJob<T, DIM>& job,
     const int e0 = ti.ext[0], e1 = ti.ext[1];
     const int ncell = e0 * e1 * ti.ext[2];
     const float r0 = 1.0f / static_cast<float>(e0), r1 = 1.0f / static_cast<float>(e1);
+    // grids below 2^32 cells whose (y, z) plane count fits 24 bits index in
+    // 32 bits with the full-rate 24-bit multiplier
+    const bool narrow = static_cast<uint64_t>(g.n[0]) * static_cast<uint64_t>(g.n[1]) *
+                                static_cast<uint64_t>(g.n[2]) < (uint64_t{1} << 32) &&
+                        g.n[1] * g.n[2] < (1 << 24);
     for (int base = threadIdx.x; base < ncell; base += kInterpThreads<T> * kStage) {
       Cplx<T> v[kStage];
       int dst[kStage];
@@ -258,10 +263,12 @@ __device__ __forceinline__ void interp_subproblem(const InterpJob<T, DIM>& job,
           const int gx = wrap_once(ti.org[0] + i0, g.n[0]);
           const int gy = DIM >= 2 ? wrap_once(ti.org[1] + i1, g.n[1]) : 0;
           const int gz = DIM >= 3 ? wrap_once(ti.org[2] + i2, g.n[2]) : 0;
-          v[k] = grid[(static_cast<std::size_t>(gz) * static_cast<std::size_t>(g.n[1]) + gy) *
-                          static_cast<std::size_t>(g.n[0]) +
-                      gx];
-          dst[k] = (i2 * ti.t1 + i1) * ti.t0 + i0;
+          const int row = __mul24(gz, g.n[1]) + gy;
+          v[k] = narrow ? grid[__umul24(static_cast<unsigned>(row), static_cast<unsigned>(g.n[0])) +
+                               static_cast<unsigned>(gx)]
+                        : grid[static_cast<std::size_t>(row) * static_cast<std::size_t>(g.n[0]) +
+                               static_cast<std::size_t>(gx)];
+          dst[k] = __mul24(__mul24(i2, ti.t1) + i1, ti.t0) + i0;
         }
       }
 #pragma unroll
