@@ -693,10 +693,10 @@ def run_uniform(args):
     for d, nb in enumerate(info["bin_size"]):
         tile_cells *= nb + ns
     reduce_bytes = info["num_subproblems"] * tile_cells * 8 + G * 8
-    # LDS work of the algorithm: spreading adds re and im (two 4-byte integer
-    # atomics) into each of ns^3 cells per point; interpolation reads each of
-    # ns^3 cells per point (one 8-byte complex read)
-    lds_spread = m * ns ** 3 * 2 * 4
+    # LDS work of the algorithm: spreading adds re and im (one packed 8-byte
+    # integer atomic) into each of ns^3 cells per point; interpolation reads
+    # each of ns^3 cells per point (one 8-byte complex read)
+    lds_spread = m * ns ** 3 * 8
     lds_interp = m * ns ** 3 * 8
     prof = _profile_counters()
     legs = {
@@ -716,7 +716,7 @@ def run_uniform(args):
     lds_bps = dom["lds_bytes"] / (dom["ms"] * 1e-3)
     lds = {"achieved": lds_bps / 1e12, "peak": LDS_PEAK / 1e12, "unit": "TB/s",
            "frac": lds_bps / LDS_PEAK, "lds_bytes_per_launch": dom["lds_bytes"],
-           "note": ("algorithmic LDS data bytes (spreading: 2 x 4-byte atomics per point and "
+           "note": ("algorithmic LDS data bytes (spreading: one packed 8-byte atomic per point and "
                     "cell, each a read-modify-write of its bank; interpolation: one 8-byte "
                     "read per point and cell) / launch time, against 128 B/clk/CU x 256 CUs "
                     "x 2.4 GHz")}
