@@ -12,6 +12,7 @@
 // Points are read in a row-sorted view (sorted by the first kernel row i0y),
 // so the points of a band are one or two contiguous ranges; their strengths
 // are first gathered into that order so that the bands read them coalesced.
+#include "chunk_rank.hpp"
 #include "lds_fft.hpp"
 
 namespace nnf {
@@ -45,17 +46,16 @@ __device__ inline void rec_start(const PointRec<T, 2>& r, const BinGeom& g, int 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kRowBlock)
+__global__ void __launch_bounds__(kChunkThreads)
     row_count_kernel(const PointRec<T, 2>* __restrict__ recs, std::size_t m, BinGeom g, int ns,
                      uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
-  for (std::size_t p = threadIdx.x + static_cast<std::size_t>(blockIdx.x) * kRowBlock; p < m;
-       p += static_cast<std::size_t>(gridDim.x) * kRowBlock) {
+  // the records are bin-sorted: a chunk of them starts on a few dozen rows
+  chunk_rank(m, count, rank, [&](std::size_t p) {
     int i0x, i0y;
     rec_start(recs[p], g, ns, i0x, i0y);
     // i0y = ceil(y - ns/2) lies in [-ns/2, n1 - 1]
-    const int row = i0y < 0 ? i0y + g.n[1] : i0y;
-    rank[p] = atomicAdd(&count[row], 1u);
-  }
+    return static_cast<uint32_t>(i0y < 0 ? i0y + g.n[1] : i0y);
+  });
 }
 
 template <typename T>
@@ -477,7 +477,7 @@ template <typename T>
 void launch_row_count(const PointRec<T, 2>* recs, std::size_t m, const BinGeom& g, int ns,
                       uint32_t* count, uint32_t* rank, hipStream_t stream) {
   if (m == 0) return;
-  hipLaunchKernelGGL((row_count_kernel<T>), dim3(blocks_for(m, kRowBlock)), dim3(kRowBlock), 0,
+  hipLaunchKernelGGL((row_count_kernel<T>), dim3(chunk_rank_blocks(m)), dim3(kChunkThreads), 0,
                      stream, recs, m, g, ns, count, rank);
   check_launch("row_count_kernel");
 }

@@ -7,6 +7,7 @@
 // is stored split into an integer kernel start cell and an fp32 Horner
 // argument, and the bins are cut into bounded subproblems for the spreading /
 // interpolation kernels.
+#include "chunk_rank.hpp"
 #include "kernels.hpp"
 
 namespace nnf {
@@ -56,70 +57,17 @@ __device__ inline uint32_t point_bin(const Loc<TC>& loc, std::size_t i, const Fo
 }
 
 // Counting pass of the bin sort: rank[i] = position of point i among the
-// points of its bin (count[bin] ends as the bin's size). A workgroup takes a
-// chunk of kCountChunk consecutive points and first counts them per bin in an
-// LDS hash table (linear probing; at most kCountChunk keys in kHashSlots =
-// 2 kCountChunk slots, so probing always ends), each point taking its rank
-// within the chunk from an LDS atomic; then one global atomic per distinct bin
-// of the chunk reserves the chunk's range of the bin. Clustered points put up
-// to ~1e5 points into one bin, whose per-point global atomics on one address
-// serialise; a chunk of them touches a few hundred bins.
-constexpr int kCountThreads = 256;
-constexpr int kCountPerThread = 8;
-constexpr int kCountChunk = kCountThreads * kCountPerThread;
-constexpr int kHashSlots = 2 * kCountChunk;
-constexpr int kHashBits = 12;
-static_assert((1 << kHashBits) == kHashSlots);
-constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
-
+// points of its bin (hashed per-chunk counts, one global atomic per distinct
+// bin of a chunk).
 template <typename TC, int DIM>
-__global__ void __launch_bounds__(kCountThreads)
+__global__ void __launch_bounds__(kChunkThreads)
     bin_count_kernel(Loc<TC> loc, std::size_t m, FoldParams fp, BinGeom g, int ns,
                      uint32_t* __restrict__ count, uint32_t* __restrict__ rank) {
-  __shared__ uint32_t key[kHashSlots];
-  __shared__ uint32_t cnt[kHashSlots];
-  for (std::size_t c0 = static_cast<std::size_t>(blockIdx.x) * kCountChunk; c0 < m;
-       c0 += static_cast<std::size_t>(gridDim.x) * kCountChunk) {
-    for (int s = threadIdx.x; s < kHashSlots; s += kCountThreads) {
-      key[s] = kEmptyKey;
-      cnt[s] = 0;
-    }
-    __syncthreads();
-    int slot[kCountPerThread];
-    uint32_t local[kCountPerThread];
-#pragma unroll
-    for (int k = 0; k < kCountPerThread; ++k) {
-      const std::size_t i = c0 + static_cast<std::size_t>(k * kCountThreads) + threadIdx.x;
-      slot[k] = -1;
-      local[k] = 0;
-      if (i < m) {
-        double hx[3];
-        int rel[3];
-        const uint32_t bin = point_bin<TC, DIM>(loc, i, fp, g, ns, hx, rel);
-        int s = static_cast<int>((bin * 2654435761u) >> (32 - kHashBits));
-        for (int probe = 0; probe < kHashSlots; ++probe) {
-          const uint32_t prev = atomicCAS(&key[s], kEmptyKey, bin);
-          if (prev == kEmptyKey || prev == bin) break;
-          s = (s + 1) & (kHashSlots - 1);
-        }
-        slot[k] = s;
-        local[k] = atomicAdd(&cnt[s], 1u);
-      }
-    }
-    __syncthreads();
-    // the chunk's range of each of its bins; the slot then holds its base
-    for (int s = threadIdx.x; s < kHashSlots; s += kCountThreads) {
-      const uint32_t n = cnt[s];
-      if (n != 0) cnt[s] = atomicAdd(&count[key[s]], n);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kCountPerThread; ++k) {
-      const std::size_t i = c0 + static_cast<std::size_t>(k * kCountThreads) + threadIdx.x;
-      if (slot[k] >= 0) rank[i] = cnt[slot[k]] + local[k];
-    }
-    __syncthreads();  // the table is cleared for the next chunk
-  }
+  chunk_rank(m, count, rank, [&](std::size_t i) {
+    double hx[3];
+    int rel[3];
+    return point_bin<TC, DIM>(loc, i, fp, g, ns, hx, rel);
+  });
 }
 
 // TC: coordinate type read; T: precision of the point records written.
@@ -380,8 +328,8 @@ template <typename TC, int DIM>
 void launch_bin_count(Loc<TC> loc, std::size_t m, const FoldParams& fp, const BinGeom& g, int ns,
                       uint32_t* count, uint32_t* rank, hipStream_t stream) {
   if (m == 0) return;
-  hipLaunchKernelGGL((bin_count_kernel<TC, DIM>), dim3(grid_for(m, kCountChunk, 8192u)),
-                     dim3(kCountThreads), 0, stream, loc, m, fp, g, ns, count, rank);
+  hipLaunchKernelGGL((bin_count_kernel<TC, DIM>), dim3(chunk_rank_blocks(m)),
+                     dim3(kChunkThreads), 0, stream, loc, m, fp, g, ns, count, rank);
   check_launch("bin_count_kernel");
 }
 
