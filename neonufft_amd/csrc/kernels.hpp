@@ -276,6 +276,43 @@ void launch_reduce_rows_fft(const RowFftJob<T>& job, int ns, hipStream_t stream)
 template <typename T>
 void launch_expand_rows_ifft(const RowFftJob<T>& job, hipStream_t stream);
 
+// ---- rows.hip: the y and z passes of a 3D pruned FFT in LDS --------------
+// For power-of-two n1, n2 >= 32. Between h[(z N0 + x') n1 + y] and
+// k[(y' N0 + x') n2 + z] (y' < N1 retained), the passes the pruned FFT runs
+// as rocFFT passes joined by transposes, each in one kernel:
+//   type 1: h -> y-FFT -> k (retained y'), then k -> z-FFT -> deconvolved
+//           modes in the caller's order and strides;
+//   type 2: caller's modes -> deconvolved, zero-padded z rows -> z-FFT -> k,
+//           then k -> zero-padded y rows -> y-FFT -> h.
+// A workgroup transforms `rows` (a power of two) rows of one axis with LDS
+// rows of pitch n + 8.
+template <typename T>
+struct AxisFftJob {
+  Cplx<T>* h;
+  Cplx<T>* k;
+  Cplx<T>* modes_out;         // type 1: caller's modes written
+  const Cplx<T>* modes_in;    // type 2: caller's modes read
+  const T* fs[3];        // deconvolution factors per axis, by |mode|
+  long long st[3];       // caller's mode strides
+  int order_fft;
+  const Cplx<T>* tw;     // twiddles (tw[N/2 + q]) up to max(n1, n2)
+  int N0, N1, N2, n1, n2, log2n1, log2n2;
+  int rows_y, log2rows_y, rows_z, log2rows_z;
+};
+// Rows per workgroup of an axis pass over lines of n cells (0 = too long).
+inline int axis_fft_rows(IntType n, std::size_t tbytes, std::size_t budget) {
+  for (int rows = 16; rows >= 1; rows /= 2) {
+    if (2 * static_cast<std::size_t>(rows) * static_cast<std::size_t>(n + 8) * tbytes <= budget) {
+      return rows;
+    }
+  }
+  return 0;
+}
+template <typename T>
+void launch_axis_fft_type1(const AxisFftJob<T>& job, hipStream_t stream);
+template <typename T>
+void launch_axis_fft_type2(const AxisFftJob<T>& job, hipStream_t stream);
+
 // ---- grid.hip: deconvolution factors, up/down-sampling, phases -----------
 template <typename T>
 void launch_fill_zero(Cplx<T>* p, std::size_t count, hipStream_t stream);

@@ -187,7 +187,19 @@ public:
     // to the zero padding (type 2); NNF_SPREAD_TILES keeps the separate
     // reduction, rocFFT x pass and transposes
     if (tune_.spread_mode != kSpreadTiles) row_rows_ = row_fft_rows(DIM, geom_, sizeof(T));
-    if ((DIM == 2 && band_.fft) || row_rows_ > 0) upload_twiddles<T>(tw_, n_[0], sign_, stream_, alloc_);
+    // 3D with the row kernels: the y and z passes in LDS too, when both
+    // extents are powers of two >= 32
+    IntType tw_len = n_[0];
+    if constexpr (DIM == 3) {
+      auto pow2 = [](IntType v) { return v >= 32 && (v & (v - 1)) == 0; };
+      if (row_rows_ > 0 && pow2(n_[1]) && pow2(n_[2])) {
+        axis_rows_y_ = axis_fft_rows(n_[1], sizeof(T), kBandBudget);
+        axis_rows_z_ = axis_fft_rows(n_[2], sizeof(T), kBandBudget);
+        if (axis_rows_y_ == 0 || axis_rows_z_ == 0) axis_rows_y_ = axis_rows_z_ = 0;
+        if (axis_rows_y_ > 0) tw_len = std::max({n_[0], n_[1], n_[2]});
+      }
+    }
+    if ((DIM == 2 && band_.fft) || row_rows_ > 0) upload_twiddles<T>(tw_, tw_len, sign_, stream_, alloc_);
     set_points(num_nu, loc);
   }
 
@@ -350,7 +362,11 @@ public:
         // tiles -> retained x modes in one pass, then the remaining passes
         spread_tiles(in);
         launch_reduce_rows_fft<T, DIM>(row_job(), ks_.ns, stream_);
-        pfft_.forward_from_h(fs, order_fft(), out, nullptr, out_strides, stream_);
+        if (axis_rows_y_ > 0) {
+          launch_axis_fft_type1<T>(axis_job(out, nullptr, out_strides), stream_);
+        } else {
+          pfft_.forward_from_h(fs, order_fft(), out, nullptr, out_strides, stream_);
+        }
         return;
       }
     }
@@ -373,7 +389,11 @@ public:
       fft_.execute(grid_.data());
     } else {
       if (row_rows_ > 0) {
-        pfft_.backward_to_h(in, nullptr, in_strides, fs, order_fft(), stream_);
+        if (axis_rows_y_ > 0) {
+          launch_axis_fft_type2<T>(axis_job(nullptr, in, in_strides), stream_);
+        } else {
+          pfft_.backward_to_h(in, nullptr, in_strides, fs, order_fft(), stream_);
+        }
         launch_expand_rows_ifft<T>(row_job(), stream_);
       } else {
         pfft_.backward(in, nullptr, in_strides, fs, order_fft(), grid_.data(), stream_);
@@ -413,6 +433,32 @@ private:
     if (!points_valid_) {
       throw neonufft::InputError("Neonufft:: no valid points (the last set_points failed)");
     }
+  }
+
+  AxisFftJob<T> axis_job(Cplx<T>* out, const Cplx<T>* in, const IntType* st) const {
+    AxisFftJob<T> j{};
+    j.h = pfft_.h();
+    j.k = pfft_.k();
+    j.modes_out = out;
+    j.modes_in = in;
+    for (int d = 0; d < 3; ++d) {
+      j.fs[d] = fs_[d].data();
+      j.st[d] = static_cast<long long>(st[d]);
+    }
+    j.order_fft = order_fft();
+    j.tw = tw_.data();
+    j.N0 = static_cast<int>(modes_[0]);
+    j.N1 = static_cast<int>(modes_[1]);
+    j.N2 = static_cast<int>(modes_[2]);
+    j.n1 = static_cast<int>(n_[1]);
+    j.n2 = static_cast<int>(n_[2]);
+    j.log2n1 = log2_exact(n_[1]);
+    j.log2n2 = log2_exact(n_[2]);
+    j.rows_y = axis_rows_y_;
+    j.log2rows_y = log2_exact(axis_rows_y_);
+    j.rows_z = axis_rows_z_;
+    j.log2rows_z = log2_exact(axis_rows_z_);
+    return j;
   }
 
   RowFftJob<T> row_job() const {
@@ -471,6 +517,8 @@ private:
   BinGeom geom_{};
   int band_rows_ = 0;             // 2D row-band spreading (0 = subgrid tiles)
   int row_rows_ = 0;              // pencil height of the row kernels (0 = not used)
+  int axis_rows_y_ = 0;           // 3D: lines per workgroup of the LDS y pass (0 = rocFFT)
+  int axis_rows_z_ = 0;           // 3D: lines per workgroup of the LDS z pass
   bool bands_on_ = false;         // row bands used for the current points
   bool points_valid_ = false;     // the last set_points succeeded
   std::vector<uint32_t> row_host_;  // row offsets of the view (balance check)

@@ -75,6 +75,9 @@ public:
   // (row-band spreading or the row reduction with their own x-FFT) continues
   // with forward_from_h; type 2 stops there with backward_to_h.
   Cplx<T>* h() const { return h_.data(); }
+  // 3D: intermediate after the y pass, per grid N1 retained y' x N0 x' lines
+  // of n2 cells, z fastest (k[(y' N0 + x') n2 + z]).
+  Cplx<T>* k() const { return k_.data(); }
 
   // Type 1 from a filled h() (x-FFT done, retained x modes written there by
   // the row-band or row-reduction kernels): the remaining passes and the

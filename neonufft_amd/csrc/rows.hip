@@ -16,6 +16,7 @@
 // (src/neonufft/gpu/plan_impl.hpp:91-136: spread, in-place FFT, downsample;
 // upsample, in-place FFT, interpolate), up to rounding.
 #include "lds_fft.hpp"
+#include "mode_index.hpp"
 
 namespace nnf {
 
@@ -281,6 +282,214 @@ __global__ void __launch_bounds__(kRowThreads<T>) expand_rows_ifft_kernel(RowFft
   }
 }
 
+// ---------------------------------------------------------------------------
+// y and z passes of the 3D pruned FFT (AxisFftJob). Each workgroup loads
+// `rows` lines of one axis into LDS rows (natural order, zero padding where
+// modes are absent), transforms them with the same decimation-in-frequency
+// FFT as the x rows (output at bit-reversed positions) and writes the next
+// layout directly, so the rocFFT pass, the transpose and the deconvolution
+// pass become one read and one write of the data.
+// ---------------------------------------------------------------------------
+
+// Moves `total` items global -> LDS: each lane issues the loads of up to
+// kBatch items (clamped to a valid item, so none sits behind a branch)
+// before it stores any; a loop that stores each load before issuing the next
+// waits on every load in turn.
+template <int NT, typename Load, typename Store>
+__device__ inline void batched_copy(int total, Load load, Store store) {
+  constexpr int kBatch = 8;
+  for (int i0 = static_cast<int>(threadIdx.x); i0 < total; i0 += NT * kBatch) {
+    decltype(load(0)) v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = load(min(i0 + u * NT, total - 1));
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (i0 + u * NT < total) store(i0 + u * NT, v[u]);
+    }
+  }
+}
+
+// Type 1: lines y of h at one x' and rows z0 .. z0 + rk; the retained y' go
+// to k[(y' N0 + x') n2 + z] (consecutive lanes: consecutive z of one y').
+template <typename T>
+__global__ void __launch_bounds__(kRowThreads<T>) y_keep_kernel(AxisFftJob<T> job) {
+  constexpr int NT = kRowThreads<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int R = job.rows_y, n1 = job.n1, n2 = job.n2, N0 = job.N0;
+  const int pitch = n1 + 8;
+  T* fre = reinterpret_cast<T*>(smem_raw);
+  T* fim = fre + R * pitch;
+  const int b = static_cast<int>(blockIdx.x);
+  const int xp = b % N0;
+  const int z0 = (b / N0) * R;
+  const int rk = min(R, n2 - z0);
+  const Cplx<T>* __restrict__ src = job.h + static_cast<std::size_t>(xp) * n1;
+  const std::size_t zs = static_cast<std::size_t>(N0) * n1;
+  const int l1 = job.log2n1;
+  batched_copy<NT>(
+      rk << l1,
+      [&](int i) { return src[static_cast<std::size_t>(z0 + (i >> l1)) * zs + (i & (n1 - 1))]; },
+      [&](int i, const Cplx<T>& v) {
+        const int o = (i >> l1) * pitch + (i & (n1 - 1));
+        fre[o] = v.x;
+        fim[o] = v.y;
+      });
+  __syncthreads();
+  lds_fft_x<T, NT, false>(fre, fim, rk, pitch, job.log2n1, T(1), job.tw);
+  const int sh = 32 - job.log2n1;
+  const int lr = job.log2rows_y;
+  Cplx<T>* __restrict__ dst = job.k + static_cast<std::size_t>(xp) * n2 + z0;
+  const std::size_t ys = static_cast<std::size_t>(N0) * n2;
+  for (int i = threadIdx.x; i < (job.N1 << lr); i += NT) {
+    const int r = i & (R - 1), yp = i >> lr;
+    if (r >= rk) continue;
+    const int f = ret_fine(yp, n1, job.N1);
+    const int pos = r * pitch + static_cast<int>(__brev(static_cast<unsigned>(f)) >> sh);
+    dst[static_cast<std::size_t>(yp) * ys + r] = Cplx<T>{fre[pos], fim[pos]};
+  }
+}
+
+// Type 1: lines z of k at one y' and x' = x0 .. x0 + rk; the retained z'
+// deconvolved into the caller's modes (consecutive lanes: consecutive x').
+template <typename T>
+__global__ void __launch_bounds__(kRowThreads<T>) z_out_kernel(AxisFftJob<T> job) {
+  constexpr int NT = kRowThreads<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int R = job.rows_z, n2 = job.n2, N0 = job.N0, N1 = job.N1, N2 = job.N2;
+  const int pitch = n2 + 8;
+  T* fre = reinterpret_cast<T*>(smem_raw);
+  T* fim = fre + R * pitch;
+  const int groups = (N0 + R - 1) / R;
+  const int b = static_cast<int>(blockIdx.x);
+  const int yp = b / groups;
+  const int x0 = (b - yp * groups) * R;
+  const int rk = min(R, N0 - x0);
+  const Cplx<T>* __restrict__ src = job.k + (static_cast<std::size_t>(yp) * N0 + x0) * n2;
+  const int l2 = job.log2n2;
+  batched_copy<NT>(
+      rk << l2, [&](int i) { return src[i]; },
+      [&](int i, const Cplx<T>& v) {
+        const int o = (i >> l2) * pitch + (i & (n2 - 1));
+        fre[o] = v.x;
+        fim[o] = v.y;
+      });
+  __syncthreads();
+  lds_fft_x<T, NT, false>(fre, fim, rk, pitch, job.log2n2, T(1), job.tw);
+  const int sh = 32 - job.log2n2;
+  const int lr = job.log2rows_z;
+  const int km = ret_mode(yp, N1);
+  const T fm = job.fs[1][km < 0 ? -km : km];
+  const long long om = static_cast<long long>(order_index(km, N1, job.order_fft)) * job.st[1];
+  Cplx<T>* __restrict__ out = job.modes_out;
+  for (int i = threadIdx.x; i < (N2 << lr); i += NT) {
+    const int r = i & (R - 1), lp = i >> lr;
+    if (r >= rk) continue;
+    const int kx = ret_mode(x0 + r, N0), kl = ret_mode(lp, N2);
+    const T fac = job.fs[0][kx < 0 ? -kx : kx] * fm * job.fs[2][kl < 0 ? -kl : kl];
+    const int f = ret_fine(lp, n2, N2);
+    const int pos = r * pitch + static_cast<int>(__brev(static_cast<unsigned>(f)) >> sh);
+    out[static_cast<long long>(order_index(kx, N0, job.order_fft)) * job.st[0] + om +
+        static_cast<long long>(order_index(kl, N2, job.order_fft)) * job.st[2]] =
+        Cplx<T>{fre[pos] * fac, fim[pos] * fac};
+  }
+}
+
+// Type 2: the caller's modes of one y' and x' = x0 .. x0 + rk, deconvolved,
+// at their z positions of zero-padded lines; the transformed lines go to k.
+template <typename T>
+__global__ void __launch_bounds__(kRowThreads<T>) z_in_kernel(AxisFftJob<T> job) {
+  constexpr int NT = kRowThreads<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int R = job.rows_z, n2 = job.n2, N0 = job.N0, N1 = job.N1, N2 = job.N2;
+  const int pitch = n2 + 8;
+  T* fre = reinterpret_cast<T*>(smem_raw);
+  T* fim = fre + R * pitch;
+  const int groups = (N0 + R - 1) / R;
+  const int b = static_cast<int>(blockIdx.x);
+  const int yp = b / groups;
+  const int x0 = (b - yp * groups) * R;
+  const int rk = min(R, N0 - x0);
+  for (int i = threadIdx.x; i < 2 * R * pitch; i += NT) fre[i] = T(0);
+  __syncthreads();
+  const int lr = job.log2rows_z;
+  const int km = ret_mode(yp, N1);
+  const T fm = job.fs[1][km < 0 ? -km : km];
+  const long long om = static_cast<long long>(order_index(km, N1, job.order_fft)) * job.st[1];
+  const Cplx<T>* __restrict__ in = job.modes_in;
+  // items (r fastest, then z'); rows r >= rk repeat row rk - 1 and are dropped
+  batched_copy<NT>(
+      N2 << lr,
+      [&](int i) {
+        const int r = min(i & (R - 1), rk - 1), lp = i >> lr;
+        const int kx = ret_mode(x0 + r, N0), kl = ret_mode(lp, N2);
+        const T fac = fm * job.fs[0][kx < 0 ? -kx : kx] * job.fs[2][kl < 0 ? -kl : kl];
+        const Cplx<T> a =
+            in[static_cast<long long>(order_index(kx, N0, job.order_fft)) * job.st[0] + om +
+               static_cast<long long>(order_index(kl, N2, job.order_fft)) * job.st[2]];
+        return Cplx<T>{a.x * fac, a.y * fac};
+      },
+      [&](int i, const Cplx<T>& v) {
+        const int r = i & (R - 1), lp = i >> lr;
+        if (r >= rk) return;
+        const int o = r * pitch + ret_fine(lp, n2, N2);
+        fre[o] = v.x;
+        fim[o] = v.y;
+      });
+  __syncthreads();
+  lds_fft_x<T, NT, false>(fre, fim, rk, pitch, job.log2n2, T(1), job.tw);
+  const int sh = 32 - job.log2n2;
+  Cplx<T>* __restrict__ dst = job.k + (static_cast<std::size_t>(yp) * N0 + x0) * n2;
+  for (int i = threadIdx.x; i < (rk << job.log2n2); i += NT) {
+    const int r = i >> job.log2n2, z = i & (n2 - 1);
+    const int pos = r * pitch + static_cast<int>(__brev(static_cast<unsigned>(z)) >> sh);
+    dst[static_cast<std::size_t>(r) * n2 + z] = Cplx<T>{fre[pos], fim[pos]};
+  }
+}
+
+// Type 2: the retained y' of k at one x' and rows z0 .. z0 + rk at their y
+// positions of zero-padded lines; the transformed lines go to h.
+template <typename T>
+__global__ void __launch_bounds__(kRowThreads<T>) y_expand_kernel(AxisFftJob<T> job) {
+  constexpr int NT = kRowThreads<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int R = job.rows_y, n1 = job.n1, n2 = job.n2, N0 = job.N0;
+  const int pitch = n1 + 8;
+  T* fre = reinterpret_cast<T*>(smem_raw);
+  T* fim = fre + R * pitch;
+  const int b = static_cast<int>(blockIdx.x);
+  const int xp = b % N0;
+  const int z0 = (b / N0) * R;
+  const int rk = min(R, n2 - z0);
+  for (int i = threadIdx.x; i < 2 * R * pitch; i += NT) fre[i] = T(0);
+  __syncthreads();
+  const int lr = job.log2rows_y;
+  const Cplx<T>* __restrict__ src = job.k + static_cast<std::size_t>(xp) * n2 + z0;
+  const std::size_t ys = static_cast<std::size_t>(N0) * n2;
+  // items (r fastest, then y'); rows r >= rk repeat row rk - 1 and are dropped
+  batched_copy<NT>(
+      job.N1 << lr,
+      [&](int i) {
+        return src[static_cast<std::size_t>(i >> lr) * ys + min(i & (R - 1), rk - 1)];
+      },
+      [&](int i, const Cplx<T>& v) {
+        const int r = i & (R - 1);
+        if (r >= rk) return;
+        const int o = r * pitch + ret_fine(i >> lr, n1, job.N1);
+        fre[o] = v.x;
+        fim[o] = v.y;
+      });
+  __syncthreads();
+  lds_fft_x<T, NT, false>(fre, fim, rk, pitch, job.log2n1, T(1), job.tw);
+  const int sh = 32 - job.log2n1;
+  Cplx<T>* __restrict__ dst = job.h + static_cast<std::size_t>(xp) * n1;
+  const std::size_t zs = static_cast<std::size_t>(N0) * n1;
+  for (int i = threadIdx.x; i < (rk << job.log2n1); i += NT) {
+    const int r = i >> job.log2n1, y = i & (n1 - 1);
+    const int pos = r * pitch + static_cast<int>(__brev(static_cast<unsigned>(y)) >> sh);
+    dst[static_cast<std::size_t>(z0 + r) * zs + y] = Cplx<T>{fre[pos], fim[pos]};
+  }
+}
+
 }  // namespace
 
 template <typename T, int DIM>
@@ -310,5 +519,36 @@ template void launch_reduce_rows_fft<double, 2>(const RowFftJob<double>&, int, h
 template void launch_reduce_rows_fft<double, 3>(const RowFftJob<double>&, int, hipStream_t);
 template void launch_expand_rows_ifft<float>(const RowFftJob<float>&, hipStream_t);
 template void launch_expand_rows_ifft<double>(const RowFftJob<double>&, hipStream_t);
+
+template <typename T>
+void launch_axis_fft_type1(const AxisFftJob<T>& job, hipStream_t stream) {
+  const std::size_t by = static_cast<std::size_t>(job.N0) * ((job.n2 + job.rows_y - 1) / job.rows_y);
+  const std::size_t bz = static_cast<std::size_t>(job.N1) * ((job.N0 + job.rows_z - 1) / job.rows_z);
+  if (by == 0 || bz == 0) return;
+  hipLaunchKernelGGL((y_keep_kernel<T>), dim3(static_cast<unsigned>(by)), dim3(kRowThreads<T>),
+                     row_fft_lds_bytes(job.rows_y, job.n1 + 8, sizeof(T), 0), stream, job);
+  check_launch("y_keep_kernel");
+  hipLaunchKernelGGL((z_out_kernel<T>), dim3(static_cast<unsigned>(bz)), dim3(kRowThreads<T>),
+                     row_fft_lds_bytes(job.rows_z, job.n2 + 8, sizeof(T), 0), stream, job);
+  check_launch("z_out_kernel");
+}
+
+template <typename T>
+void launch_axis_fft_type2(const AxisFftJob<T>& job, hipStream_t stream) {
+  const std::size_t by = static_cast<std::size_t>(job.N0) * ((job.n2 + job.rows_y - 1) / job.rows_y);
+  const std::size_t bz = static_cast<std::size_t>(job.N1) * ((job.N0 + job.rows_z - 1) / job.rows_z);
+  if (by == 0 || bz == 0) return;
+  hipLaunchKernelGGL((z_in_kernel<T>), dim3(static_cast<unsigned>(bz)), dim3(kRowThreads<T>),
+                     row_fft_lds_bytes(job.rows_z, job.n2 + 8, sizeof(T), 0), stream, job);
+  check_launch("z_in_kernel");
+  hipLaunchKernelGGL((y_expand_kernel<T>), dim3(static_cast<unsigned>(by)), dim3(kRowThreads<T>),
+                     row_fft_lds_bytes(job.rows_y, job.n1 + 8, sizeof(T), 0), stream, job);
+  check_launch("y_expand_kernel");
+}
+
+template void launch_axis_fft_type1<float>(const AxisFftJob<float>&, hipStream_t);
+template void launch_axis_fft_type1<double>(const AxisFftJob<double>&, hipStream_t);
+template void launch_axis_fft_type2<float>(const AxisFftJob<float>&, hipStream_t);
+template void launch_axis_fft_type2<double>(const AxisFftJob<double>&, hipStream_t);
 
 }  // namespace nnf
