@@ -383,12 +383,76 @@ def test_type1_clustered_rows_fall_back_to_tiles(prec):
     assert O.rel_l2(bo[0].cpu().numpy().astype(np.complex128), outs["auto"]) < same
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_axis_fft_odd_modes(prec):
+    """3D plan on a power-of-two fine grid with odd mode counts (43^3 modes
+    at upsampling 1.5: 64^3 cells), so type 1 and type 2 run the y and z
+    passes in LDS with the retained-mode maps of odd N, in FFT mode order
+    with permuted caller strides (z fastest): against the FP64 direct sum,
+    the FP64 restatement and the plan without the row kernels (rocFFT passes
+    and transposes)."""
+    modes = (43, 43, 43)
+    sigma = 1.5
+    m = 5000
+    tol = 1e-5 if prec == "f32" else 1e-8
+    sign = 1
+    rng = np.random.default_rng(4343)
+    x = points(rng, 3, m, prec)
+    c = cplx(rng, m, prec)
+    nm = int(np.prod(modes))
+    f = cplx(rng, nm, prec)  # logical array (x fastest) in FFT mode order
+    loc = [dev(v, RDT[prec]) for v in x]
+    opt = nnf.Options(tol=tol, upsampfac=sigma, order=nnf.ModeOrder.FFT)
+    fused = nnf.Plan(opt, sign, m, loc, modes)
+    sep = nnf.Plan(opt, sign, m, loc, modes, spread_mode="tiles")
+    assert tuple(fused.info()["grid"]) == (64, 64, 64)
+    st = [modes[1] * modes[2], modes[2], 1]  # z fastest
+    lin = np.arange(nm)
+    off = (lin % 43) * st[0] + ((lin // 43) % 43) * st[1] + (lin // (43 * 43)) * st[2]
+    fbuf = np.empty(nm, dtype=NPC[prec])
+    fbuf[off] = f
+    outs = {}
+    for name, plan in (("fused", fused), ("sep", sep)):
+        g1 = torch.zeros(nm, dtype=CDT[prec], device="cuda")
+        plan.transform_type_1(dev(c, CDT[prec]), g1, st)
+        g2 = torch.zeros(m, dtype=CDT[prec], device="cuda")
+        plan.transform_type_2(dev(fbuf, CDT[prec]), st, g2)
+        torch.cuda.synchronize()
+        outs[name] = (g1.cpu().numpy().astype(np.complex128)[off],
+                      g2.cpu().numpy().astype(np.complex128))
+    xs = [v.astype(np.float64) for v in x]
+    c64, f64 = c.astype(np.complex128), f.astype(np.complex128)
+    amp = amplification(modes, tol, sigma)
+    idx = sample_modes(modes, rng)
+    ref1 = O.direct_type1(xs, c64, O.mode_vectors(modes, idx, order_fft=True), sign)
+    alg1 = O.nufft1(xs, c64, modes, sign, tol, upsampfac=sigma, order_fft=True)
+    # the direct type-2 sum takes CMCL order: roll every axis by N // 2
+    f_cmcl = f64.reshape(tuple(reversed(modes)))
+    for ax, n_ax in enumerate(reversed(modes)):
+        f_cmcl = np.roll(f_cmcl, n_ax // 2, axis=ax)
+    ref2 = O.direct_type2(xs, f_cmcl.ravel(), modes, sign)
+    alg2 = O.nufft2(xs, f64, modes, sign, tol, upsampfac=sigma, order_fft=True)
+    for name, (g1, g2) in outs.items():
+        assert O.rel_l2(g1[idx], ref1) < bound(prec, O.rel_l2(alg1[idx], ref1), tol, amp), name
+        assert O.rel_l2(g1, alg1) < alg_bound(prec, tol, amp), name
+        assert O.rel_l2(g2, ref2) < bound(prec, O.rel_l2(alg2, ref2), tol, amp), name
+        assert O.rel_l2(g2, alg2) < alg_bound(prec, tol, amp), name
+    # the two FFT paths agree to rounding
+    for t in (0, 1):
+        d = O.rel_l2(outs["fused"][t], outs["sep"][t])
+        assert d < (1e-5 if prec == "f32" else 1e-12), (t, d)
+
+
 ROW_CASES = [
     # (dim, modes, M, tol, sign): fine grids with a power-of-two x extent, so
     # type 1 joins the tile reduction to the x-FFT and type 2 the zero padding
     # (x extent >= 2 bins, so the tile reduction is the bin-regular one)
     (3, (32, 12, 20), 4000, 1e-5, -1),
     (3, (64, 40, 24), 20000, 1e-6, 1),
+    # powers of two on every axis (64 x 64 x 32): the y and z passes run in
+    # LDS too (against rocFFT passes and transposes in the "tiles" plan)
+    (3, (32, 31, 16), 6000, 1e-5, -1),
     (2, (128, 50), 40000, 1e-5, 1),
     (2, (64, 200), 60000, 1e-4, -1),
 ]
