@@ -25,6 +25,24 @@ namespace {
 template <typename T>
 constexpr int kRowThreads = sizeof(T) == 4 ? 512 : 256;
 
+// Moves `total` items global -> LDS: each lane issues the loads of up to
+// kBatch items (clamped to a valid item, so none sits behind a branch)
+// before it stores any; a loop that stores each load before issuing the next
+// waits on every load in turn.
+template <int NT, typename Load, typename Store>
+__device__ inline void batched_copy(int total, Load load, Store store) {
+  constexpr int kBatch = 8;
+  for (int i0 = static_cast<int>(threadIdx.x); i0 < total; i0 += NT * kBatch) {
+    decltype(load(0)) v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = load(min(i0 + u * NT, total - 1));
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (i0 + u * NT < total) store(i0 + u * NT, v[u]);
+    }
+  }
+}
+
 // Pencil of workgroup blockIdx.x: rows [y0, y0 + rk) of plane z. Pencils
 // next to each other in y and z (which read the same halo tiles) go to one
 // XCD and so meet in its L2.
@@ -258,15 +276,20 @@ __global__ void __launch_bounds__(kRowThreads<T>) expand_rows_ifft_kernel(RowFft
   const int P = N0 - N0 / 2;
   const int lr = job.log2rows;
   const Cplx<T>* __restrict__ hz = job.h + static_cast<std::size_t>(pc.z) * N0 * n1;
-  for (int i = threadIdx.x; i < (N0 << lr); i += NT) {
-    const int r = i & (job.rows - 1);
-    const int xp = i >> lr;
-    if (r >= pc.rk) continue;
-    const int f = xp < P ? xp : xp + n0 - N0;
-    const Cplx<T> v = hz[static_cast<std::size_t>(xp) * n1 + pc.y0 + r];
-    fre[r * job.pitch + f] = v.x;
-    fim[r * job.pitch + f] = v.y;
-  }
+  // items (r fastest, then x'); rows r >= rk repeat row rk - 1 and are dropped
+  batched_copy<NT>(
+      N0 << lr,
+      [&](int i) {
+        return hz[static_cast<std::size_t>(i >> lr) * n1 + pc.y0 + min(i & (job.rows - 1), pc.rk - 1)];
+      },
+      [&](int i, const Cplx<T>& v) {
+        const int r = i & (job.rows - 1);
+        if (r >= pc.rk) return;
+        const int xp = i >> lr;
+        const int o = r * job.pitch + (xp < P ? xp : xp + n0 - N0);
+        fre[o] = v.x;
+        fim[o] = v.y;
+      });
   __syncthreads();
 
   lds_fft_x<T, NT, false>(fre, fim, pc.rk, job.pitch, job.log2n0, T(1), job.tw);
@@ -290,24 +313,6 @@ __global__ void __launch_bounds__(kRowThreads<T>) expand_rows_ifft_kernel(RowFft
 // layout directly, so the rocFFT pass, the transpose and the deconvolution
 // pass become one read and one write of the data.
 // ---------------------------------------------------------------------------
-
-// Moves `total` items global -> LDS: each lane issues the loads of up to
-// kBatch items (clamped to a valid item, so none sits behind a branch)
-// before it stores any; a loop that stores each load before issuing the next
-// waits on every load in turn.
-template <int NT, typename Load, typename Store>
-__device__ inline void batched_copy(int total, Load load, Store store) {
-  constexpr int kBatch = 8;
-  for (int i0 = static_cast<int>(threadIdx.x); i0 < total; i0 += NT * kBatch) {
-    decltype(load(0)) v[kBatch];
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) v[u] = load(min(i0 + u * NT, total - 1));
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      if (i0 + u * NT < total) store(i0 + u * NT, v[u]);
-    }
-  }
-}
 
 // Type 1: lines y of h at one x' and rows z0 .. z0 + rk; the retained y' go
 // to k[(y' N0 + x') n2 + z] (consecutive lanes: consecutive z of one y').
