@@ -8,7 +8,8 @@
 //   G[b][z][y][x] --x-FFT--> compact-transpose --> H[b][z][x'][y]
 //   --y-FFT--> compact-transpose --> K[b][y'][x'][z] --z-FFT--> modes
 // (2D stops after H). Type 2 runs the mirror image with zero-padding
-// transposes.
+// transposes. 3D plans on power-of-two grids replace the passes after H with
+// the LDS axis kernels of rows.hip and use only the h() and k() arrays.
 #pragma once
 
 #include <memory>

@@ -12,6 +12,9 @@
 //  - type 2: expand_rows_ifft_kernel reads the retained x modes of the pencil
 //    from h, zero-pads them to n0 in LDS, transforms along x and writes whole
 //    fine-grid rows.
+// In 3D with power-of-two y and z extents, the y and z passes also run in
+// LDS here (y_keep / z_out for type 1, z_in / y_expand for type 2), so no
+// rocFFT pass, transpose or separate deconvolution pass is left.
 // The results equal those of the full transform the reference runs
 // (src/neonufft/gpu/plan_impl.hpp:91-136: spread, in-place FFT, downsample;
 // upsample, in-place FFT, interpolate), up to rounding.
