@@ -238,43 +238,80 @@ __device__ __forceinline__ void interp_subproblem(const InterpJob<T, DIM>& job,
   // cells flattened (x fastest, so reads stay coalesced along rows) and each
   // lane issues kStage independent loads before storing any: one 38-cell row
   // per wave and step left 40 % of the lanes idle and exposed the full load
-  // latency once per row.
+  // latency once per row. A lane's cell (x, y, z) advances by a whole
+  // workgroup of cells per step with two carries instead of a division per
+  // cell, and tiles that do not cross the grid's edge skip the periodic wrap.
   {
+    constexpr int S = kInterpThreads<T>;
     constexpr int kStage = 4;
-    const int e0 = ti.ext[0], e1 = ti.ext[1];
-    const int ncell = e0 * e1 * ti.ext[2];
-    const float r0 = 1.0f / static_cast<float>(e0), r1 = 1.0f / static_cast<float>(e1);
+    const int e0 = ti.ext[0], e1 = ti.ext[1], e2 = ti.ext[2];
+    // advance of (x, y, z) per step of S cells: S = drow e0 + dx, drow = dz e1 + dy
+    const int dx = S % e0, drow = S / e0;
+    const int dy = drow % e1, dz = drow / e1;
+    int x, y, z;
+    {
+      int rest;
+      x = div_rem(static_cast<int>(threadIdx.x), e0, 1.0f / static_cast<float>(e0), &rest);
+      y = div_rem(rest, e1, 1.0f / static_cast<float>(e1), &z);
+    }
+    bool interior = true;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) interior &= ti.org[d] >= 0 && ti.org[d] + ti.ext[d] <= g.n[d];
     // grids below 2^32 cells whose (y, z) plane count fits 24 bits index in
     // 32 bits with the full-rate 24-bit multiplier
     const bool narrow = static_cast<uint64_t>(g.n[0]) * static_cast<uint64_t>(g.n[1]) *
                                 static_cast<uint64_t>(g.n[2]) < (uint64_t{1} << 32) &&
                         g.n[1] * g.n[2] < (1 << 24);
-    for (int base = threadIdx.x; base < ncell; base += kInterpThreads<T> * kStage) {
-      Cplx<T> v[kStage];
-      int dst[kStage];
+    // one straight-line loop per (interior, narrow) case: a branch inside the
+    // step body makes the compiler wait for each load before the next
+    auto stage = [&](auto interior_c, auto narrow_c) __attribute__((always_inline)) {
+      constexpr bool kInterior = decltype(interior_c)::value;
+      constexpr bool kNarrow = decltype(narrow_c)::value;
+      while (z < e2) {
+        Cplx<T> v[kStage];
+        int dst[kStage];
 #pragma unroll
-      for (int k = 0; k < kStage; ++k) {
-        const int i = base + k * kInterpThreads<T>;
-        dst[k] = -1;
-        if (i < ncell) {
-          int i1, i2;
-          const int i0 = div_rem(i, e0, r0, &i1);
-          i1 = div_rem(i1, e1, r1, &i2);
-          const int gx = wrap_once(ti.org[0] + i0, g.n[0]);
-          const int gy = DIM >= 2 ? wrap_once(ti.org[1] + i1, g.n[1]) : 0;
-          const int gz = DIM >= 3 ? wrap_once(ti.org[2] + i2, g.n[2]) : 0;
-          const int row = __mul24(gz, g.n[1]) + gy;
-          v[k] = narrow ? grid[__umul24(static_cast<unsigned>(row), static_cast<unsigned>(g.n[0])) +
-                               static_cast<unsigned>(gx)]
-                        : grid[static_cast<std::size_t>(row) * static_cast<std::size_t>(g.n[0]) +
-                               static_cast<std::size_t>(gx)];
-          dst[k] = __mul24(__mul24(i2, ti.t1) + i1, ti.t0) + i0;
+        for (int k = 0; k < kStage; ++k) {
+          dst[k] = -1;
+          if (z < e2) {
+            int gx = ti.org[0] + x;
+            int gy = DIM >= 2 ? ti.org[1] + y : 0;
+            int gz = DIM >= 3 ? ti.org[2] + z : 0;
+            if constexpr (!kInterior) {
+              gx = wrap_once(gx, g.n[0]);
+              if constexpr (DIM >= 2) gy = wrap_once(gy, g.n[1]);
+              if constexpr (DIM >= 3) gz = wrap_once(gz, g.n[2]);
+            }
+            const int row = __mul24(gz, g.n[1]) + gy;
+            if constexpr (kNarrow) {
+              v[k] = grid[__umul24(static_cast<unsigned>(row), static_cast<unsigned>(g.n[0])) +
+                          static_cast<unsigned>(gx)];
+            } else {
+              v[k] = grid[static_cast<std::size_t>(row) * static_cast<std::size_t>(g.n[0]) +
+                          static_cast<std::size_t>(gx)];
+            }
+            dst[k] = __mul24(__mul24(z, ti.t1) + y, ti.t0) + x;
+          }
+          x += dx;
+          const int cx = x >= e0 ? 1 : 0;
+          x -= cx ? e0 : 0;
+          y += dy + cx;
+          const int cy = y >= e1 ? 1 : 0;
+          y -= cy ? e1 : 0;
+          z += dz + cy;
+        }
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+          if (dst[k] >= 0) tile[dst[k]] = v[k];
         }
       }
-#pragma unroll
-      for (int k = 0; k < kStage; ++k) {
-        if (dst[k] >= 0) tile[dst[k]] = v[k];
-      }
+    };
+    using yes = std::true_type;
+    using no = std::false_type;
+    if (interior) {
+      if (narrow) stage(yes{}, yes{}); else stage(yes{}, no{});
+    } else {
+      if (narrow) stage(no{}, yes{}); else stage(no{}, no{});
     }
   }
   __syncthreads();
