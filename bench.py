@@ -37,8 +37,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-# LDS: 128 bytes per clock per CU, 256 CUs, 2.4 GHz peak engine clock
-LDS_PEAK = 128 * 256 * 2.4e9
+# LDS: 256 bytes per clock per CU (twice CDNA3's 128; 16-byte ds_read_b128 was
+# measured at 221 B/clk/CU and 8-byte ds_read_b64 at 175 on this GPU, counting
+# clocks at the nominal rate), 256 CUs, 2.4 GHz peak engine clock
+LDS_PEAK = 256 * 256 * 2.4e9
+# VALU: one wave64 instruction per clock per CU (4 SIMDs x 16 lanes), 256 CUs
+VALU_PEAK = 256 * 2.4e9
 SEED = 0x4E554654
 
 
@@ -718,7 +722,7 @@ def run_uniform(args):
            "frac": lds_bps / LDS_PEAK, "lds_bytes_per_launch": dom["lds_bytes"],
            "note": ("algorithmic LDS data bytes (spreading: one packed 8-byte atomic per point and "
                     "cell, each a read-modify-write of its bank; interpolation: one 8-byte "
-                    "read per point and cell) / launch time, against 128 B/clk/CU x 256 CUs "
+                    "read per point and cell) / launch time, against 256 B/clk/CU x 256 CUs "
                     "x 2.4 GHz")}
     if prof:
         busy = {}
@@ -728,7 +732,21 @@ def run_uniform(args):
                 busy[k] = {"lds_busy_frac": ctr["lds_busy_frac"],
                            "bank_conflict_frac_of_busy": ctr.get("bank_conflict_frac_of_busy")}
         lds["rocprof"] = busy
-    bound = "lds" if lds_bps / LDS_PEAK > hbm_gbs / HBM_PEAK else "hbm"
+    # VALU issue rate of each kernel of the leg, from the committed counters
+    # (SQ_INSTS_VALU over the kernel-trace duration of the same profiled run)
+    valu = None
+    valu_frac = 0.0
+    if prof:
+        valu = {"peak_wave_instr_per_s": VALU_PEAK, "source": PROFILE_COUNTERS, "kernels": {}}
+        for k in dom["kernels"]:
+            ctr = prof.get(k, {})
+            if "sq_insts_valu" in ctr and "avg_us_kernel_trace" in ctr:
+                rate = ctr["sq_insts_valu"] / (ctr["avg_us_kernel_trace"] * 1e-6)
+                valu["kernels"][k] = {"wave_instr": ctr["sq_insts_valu"],
+                                      "wave_instr_per_s": rate, "frac": rate / VALU_PEAK}
+                valu_frac = max(valu_frac, rate / VALU_PEAK)
+    fracs = {"hbm": hbm_gbs / HBM_PEAK, "lds": lds_bps / LDS_PEAK, "valu": valu_frac}
+    bound = max(fracs, key=fracs.get)
 
     f_host = f.cpu().numpy()
     out2_host = out2.cpu().numpy()
@@ -776,6 +794,8 @@ def run_uniform(args):
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_ms": dom["ms"],
             "lds": lds,
+            "valu": valu,
+            "ceiling_fracs": fracs,
         },
         "kernels": {
             "spread_stage_ms": stage_ms,
