@@ -164,7 +164,7 @@ public:
         bands_ = bands_ && g.band.rows > 0;
         fused_ = fused_ && g.band.fft;
         band_lds_ = std::max(band_lds_, band_lds_bytes(g.band.rows, g.band.pitch, sizeof(T)));
-        tw_len_ = std::max<IntType>(tw_len_, g.n[0]);
+        tw_len_ = std::max<IntType>(tw_len_, std::max(g.n[0], g.n[1]));
       }
     }
     if constexpr (DIM != 2) bands_ = false;
@@ -434,7 +434,11 @@ private:
           g.pfft.set_stream(s);
           if constexpr (DIM == 2) {
             if (fused_ && bands_on_) {
-              g.pfft.forward_from_h(fs, order_fft(), nullptr, outs, st, s);
+              if (g.pfft.y_lds_rows() > 0) {
+                g.pfft.forward_from_h_lds(tw_.data(), fs, order_fft(), nullptr, outs, st, s);
+              } else {
+                g.pfft.forward_from_h(fs, order_fft(), nullptr, outs, st, s);
+              }
               continue;
             }
           }

@@ -199,6 +199,8 @@ public:
         if (axis_rows_y_ > 0) tw_len = std::max({n_[0], n_[1], n_[2]});
       }
     }
+    // 2D: the y pass of type 1 in LDS (power-of-two n1) uses the same table
+    if constexpr (DIM == 2) tw_len = std::max(n_[0], n_[1]);
     if ((DIM == 2 && band_.fft) || row_rows_ > 0) upload_twiddles<T>(tw_, tw_len, sign_, stream_, alloc_);
     set_points(num_nu, loc);
   }
@@ -353,7 +355,7 @@ public:
         launch_band_spread<T>(ks_.ns, nullptr, band_job(nullptr), 1, static_cast<std::size_t>(nb),
                               band_lds_bytes(band_rows_, band_.pitch, sizeof(T)), false, true,
                               ks_.es, stream_);
-        pfft_.forward_from_h(fs, order_fft(), out, nullptr, out_strides, stream_);
+        forward_from_h(fs, out, out_strides);
         return;
       }
     }
@@ -365,7 +367,7 @@ public:
         if (axis_rows_y_ > 0) {
           launch_axis_fft_type1<T>(axis_job(out, nullptr, out_strides), stream_);
         } else {
-          pfft_.forward_from_h(fs, order_fft(), out, nullptr, out_strides, stream_);
+          forward_from_h(fs, out, out_strides);
         }
         return;
       }
@@ -432,6 +434,16 @@ private:
   void require_points() const {
     if (!points_valid_) {
       throw neonufft::InputError("Neonufft:: no valid points (the last set_points failed)");
+    }
+  }
+
+  // Type 1 after the x pass (h filled): the y pass in LDS for 2D plans with a
+  // power-of-two n1, else the rocFFT passes and the deconvolution.
+  void forward_from_h(T* const* fs, Cplx<T>* out, const IntType* st) {
+    if (DIM == 2 && pfft_.y_lds_rows() > 0) {
+      pfft_.forward_from_h_lds(tw_.data(), fs, order_fft(), out, nullptr, st, stream_);
+    } else {
+      pfft_.forward_from_h(fs, order_fft(), out, nullptr, st, stream_);
     }
   }
 

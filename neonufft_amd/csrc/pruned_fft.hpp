@@ -16,6 +16,7 @@
 
 #include "fft.hpp"
 #include "kernels.hpp"
+#include "params.hpp"
 
 namespace nnf {
 
@@ -58,6 +59,10 @@ public:
       N_[d] = d < DIM ? static_cast<long long>(N[d]) : 1;
     }
     count_ = count;
+    y_rows_ = 0;
+    if (DIM == 2 && n_[1] >= 32 && (n_[1] & (n_[1] - 1)) == 0) {
+      y_rows_ = axis_fft_rows(static_cast<IntType>(n_[1]), sizeof(T), kBandBudget);
+    }
     const bool dbl = std::is_same_v<T, double>;
     IntType len = n_[0];
     px_ = FftPlan(dbl, 1, &len, sign, static_cast<IntType>(count * n_[1] * n_[2]), stream, alloc);
@@ -87,6 +92,39 @@ public:
                       const IntType* st, hipStream_t s) {
     py_.execute(h_.data());
     finish_forward(fs, order_fft, single, outs, st, s);
+  }
+
+  // 2D: workgroup rows of the y pass in LDS (power-of-two n1 >= 32 within
+  // the LDS budget), else 0.
+  int y_lds_rows() const { return y_rows_; }
+
+  // 2D type 1 from a filled h() with the y pass in LDS (y_lds_rows() > 0;
+  // twiddles tw[N/2 + q] of the plan's sign up to n1): the y-FFT and the
+  // deconvolution into the outputs read h once and write the modes once,
+  // instead of the rocFFT pass plus the transposing deconvolution.
+  void forward_from_h_lds(const Cplx<T>* tw, T* const* fs, int order_fft, Cplx<T>* single,
+                          Cplx<T>* const* outs, const IntType* st, hipStream_t s) {
+    YOut2Job<T> j{};
+    j.h = h_.data();
+    j.member_stride = N_[0] * n_[1];
+    j.single = single;
+    j.outs = outs;
+    j.fs0 = fs[0];
+    j.fs1 = fs[1];
+    j.st0 = st[0];
+    j.st1 = st[1];
+    j.order_fft = order_fft;
+    j.tw = tw;
+    j.count = count_;
+    j.N0 = static_cast<int>(N_[0]);
+    j.N1 = static_cast<int>(N_[1]);
+    j.n1 = static_cast<int>(n_[1]);
+    j.rows = y_rows_;
+    for (j.log2n1 = 0; (1LL << j.log2n1) < n_[1]; ++j.log2n1) {
+    }
+    for (j.log2rows = 0; (1 << j.log2rows) < y_rows_; ++j.log2rows) {
+    }
+    launch_y_out2<T>(j, s);
   }
 
   // Type 2 up to h(): deconvolution, zero padding and the passes after x; the
@@ -170,6 +208,7 @@ private:
 
   long long n_[3] = {1, 1, 1}, N_[3] = {1, 1, 1};
   int count_ = 0;
+  int y_rows_ = 0;
   FftPlan px_, py_, pz_;
   DeviceBuffer<Cplx<T>> h_, k_;
 };
