@@ -159,53 +159,81 @@ __device__ inline int find_job(const BandJob<T>* jobs, int njobs, std::size_t i,
   return lo;
 }
 
+// The fields of a job the gather uses.
+template <typename T>
+struct GatherJob {
+  const uint32_t* perm;
+  const uint16_t* vrow;
+  const Cplx<T>* in;
+  Cplx<T>* sorted;
+  T* rowsum;
+  T* rowmax;
+  std::size_t pt_off, m;
+};
+
+template <typename T>
+__device__ inline GatherJob<T> gather_fields(const BandJob<T>& b) {
+  return GatherJob<T>{b.perm, b.vrow, b.in, b.sorted, b.rowsum, b.rowmax, b.pt_off, b.m};
+}
+
 // sorted[p] = in[perm[p]] over the points of every job (flattened), and per
 // first kernel row the sum and the largest |Re c| + |Im c| of its points into
 // the job's (cleared) rowsum / rowmax. Consecutive lanes take consecutive view
 // points (sorted by row), so one row's points are a run of lanes: a segmented
 // wave scan combines each run and the run's last lane issues one atomic add
-// and one atomic max.
+// and one atomic max. A workgroup's loads form two dependent rounds after its
+// job's fields: the view indices and rows of all its items, then their
+// strengths. With `chunks`, workgroup b gathers the one job range chunks[b]
+// names.
 template <typename T>
 __global__ void __launch_bounds__(kRowBlock)
-    band_gather_kernel(const BandJob<T>* __restrict__ jobs, BandJob<T> single, int njobs,
-                       std::size_t total) {
-  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * (kRowBlock * kGatherItems);
+    band_gather_kernel(const BandJob<T>* __restrict__ jobs, const GatherChunk* __restrict__ chunks,
+                       BandJob<T> single, int njobs, std::size_t total) {
+  static_assert(kRowBlock * kGatherItems == kGatherChunk, "gather chunk size");
+  std::size_t base = static_cast<std::size_t>(blockIdx.x) * kGatherChunk;
+  int j = 0;
+  GatherJob<T> job = gather_fields(single);
+  if (chunks) {
+    const GatherChunk e = chunks[blockIdx.x];
+    if (e.job == 0xffffffffu) return;
+    j = static_cast<int>(e.job);
+    job = gather_fields(jobs[j]);
+    base = job.pt_off + e.first;
+    total = job.pt_off + job.m;  // this job only
+  } else if (jobs) {
+    if (base >= total) return;
+    j = find_job(jobs, njobs, base, false);
+    job = gather_fields(jobs[j]);
+  }
   if (base >= total) return;
-  int j = jobs ? find_job(jobs, njobs, base, false) : 0;
-  BandJob<T> job = jobs ? jobs[j] : single;
-  // all index loads, then all strength loads, then the stores
   Cplx<T>* dst[kGatherItems];
-  const Cplx<T>* src[kGatherItems];
-  int jk[kGatherItems];
-  uint32_t pp[kGatherItems];
+  const Cplx<T>* in[kGatherItems];
+  uint32_t idx[kGatherItems];
+  int lo[kGatherItems], jk[kGatherItems];
 #pragma unroll
   for (int k = 0; k < kGatherItems; ++k) {
     const std::size_t i = base + threadIdx.x + static_cast<std::size_t>(k) * kRowBlock;
     dst[k] = nullptr;
-    src[k] = nullptr;
+    in[k] = nullptr;
+    idx[k] = 0u;
+    lo[k] = 0;
     jk[k] = j;
-    pp[k] = 0u;
     if (i < total) {
-      while (i >= job.pt_off + job.m) job = jobs[++j];
+      while (i >= job.pt_off + job.m) job = gather_fields(jobs[++j]);
       const std::size_t p = i - job.pt_off;
       dst[k] = job.sorted + p;
-      src[k] = job.in + job.perm[p];
+      in[k] = job.in;
+      idx[k] = job.perm[p];
+      lo[k] = static_cast<int>(job.vrow[p]);
       jk[k] = j;
-      pp[k] = static_cast<uint32_t>(p);
     }
   }
   Cplx<T> v[kGatherItems];
 #pragma unroll
-  for (int k = 0; k < kGatherItems; ++k) v[k] = src[k] ? *src[k] : Cplx<T>{T(0), T(0)};
+  for (int k = 0; k < kGatherItems; ++k) v[k] = dst[k] ? in[k][idx[k]] : Cplx<T>{T(0), T(0)};
 #pragma unroll
   for (int k = 0; k < kGatherItems; ++k) {
     if (dst[k]) *dst[k] = v[k];
-  }
-  // first kernel row of every item (row table of the view)
-  int lo[kGatherItems];
-#pragma unroll
-  for (int k = 0; k < kGatherItems; ++k) {
-    lo[k] = dst[k] ? static_cast<int>((jobs ? jobs[jk[k]] : single).vrow[pp[k]]) : 0;
   }
   const int lane = static_cast<int>(threadIdx.x & 63);
 #pragma unroll
@@ -227,7 +255,7 @@ __global__ void __launch_bounds__(kRowBlock)
     }
     const uint32_t next = __shfl_down(key, 1, 64);
     if (dst[k] && (lane == 63 || next != key)) {
-      const BandJob<T>& jb = jobs ? jobs[jk[k]] : single;
+      const GatherJob<T> jb = jk[k] == j ? job : gather_fields(jobs[jk[k]]);
       atomicAdd(jb.rowsum + lo[k], s);
       max_into(jb.rowmax + lo[k], mx);
     }
@@ -254,14 +282,18 @@ __device__ inline T wave_max(T v) {
 //     written, transposed, to job.h[x' * n1 + y].
 template <typename T, int NS, bool FFT>
 __global__ void __launch_bounds__(kBandThreads<T>)
-    __attribute__((amdgpu_waves_per_eu(kBandWaves<T>))) band_spread_kernel(const BandJob<T>* __restrict__ jobs, BandJob<T> single, int njobs,
-                       EsParams es, int accumulate) {
+    __attribute__((amdgpu_waves_per_eu(kBandWaves<T>)))
+    band_spread_kernel(const BandJob<T>* __restrict__ jobs, const uint32_t* __restrict__ band_job,
+                       BandJob<T> single, int njobs, EsParams es, int accumulate) {
   using Acc = typename FixedAcc<T>::type;
   constexpr int NT = kBandThreads<T>;
   constexpr int KB = kBandBatch;
   extern __shared__ __align__(16) unsigned char smem_raw[];
 
-  const BandJob<T> job = jobs ? jobs[find_job(jobs, njobs, blockIdx.x, true)] : single;
+  const BandJob<T> job =
+      jobs ? jobs[band_job ? static_cast<int>(band_job[blockIdx.x])
+                           : find_job(jobs, njobs, blockIdx.x, true)]
+           : single;
   const int n0 = job.n0, n1 = job.n1, pitch = job.pitch;
   const int y0 = static_cast<int>(blockIdx.x - job.band_off) * job.rows;
   const int rk = min(job.rows, n1 - y0);
@@ -450,15 +482,15 @@ __global__ void __launch_bounds__(kBandThreads<T>)
 }
 
 template <typename T, int NS>
-void band_spread_ns(const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
-                    std::size_t bands, std::size_t lds, bool accumulate, bool fft,
+void band_spread_ns(const BandJob<T>* jobs, const uint32_t* band_job, const BandJob<T>& single,
+                    int njobs, std::size_t bands, std::size_t lds, bool accumulate, bool fft,
                     const EsParams& es, hipStream_t stream) {
   if (fft) {
     hipLaunchKernelGGL((band_spread_kernel<T, NS, true>), dim3(static_cast<unsigned>(bands)),
-                       dim3(kBandThreads<T>), lds, stream, jobs, single, njobs, es, 0);
+                       dim3(kBandThreads<T>), lds, stream, jobs, band_job, single, njobs, es, 0);
   } else {
     hipLaunchKernelGGL((band_spread_kernel<T, NS, false>), dim3(static_cast<unsigned>(bands)),
-                       dim3(kBandThreads<T>), lds, stream, jobs, single, njobs, es,
+                       dim3(kBandThreads<T>), lds, stream, jobs, band_job, single, njobs, es,
                        accumulate ? 1 : 0);
   }
   check_launch("band_spread_kernel");
@@ -520,25 +552,35 @@ template <typename T>
 void launch_band_gather(const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
                         std::size_t total_points, hipStream_t stream) {
   if (total_points == 0) return;
-  const std::size_t per = static_cast<std::size_t>(kRowBlock) * kGatherItems;
-  const std::size_t blocks = (total_points + per - 1) / per;
+  const std::size_t blocks = (total_points + kGatherChunk - 1) / kGatherChunk;
   if (blocks > 0x7fffffffu) throw neonufft::InputError("Neonufft:: too many points");
   hipLaunchKernelGGL((band_gather_kernel<T>), dim3(static_cast<unsigned>(blocks)),
-                     dim3(kRowBlock), 0, stream, jobs, single, njobs, total_points);
+                     dim3(kRowBlock), 0, stream, jobs, nullptr, single, njobs, total_points);
   check_launch("band_gather_kernel");
 }
 
 template <typename T>
-void launch_band_spread(int ns, const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
-                        std::size_t total_bands, std::size_t lds_bytes, bool accumulate, bool fft,
-                        const EsParams& es, hipStream_t stream) {
+void launch_band_gather_chunks(const BandJob<T>* jobs, const GatherChunk* chunks,
+                               std::size_t nchunks, hipStream_t stream) {
+  if (nchunks == 0) return;
+  if (nchunks > 0x7fffffffu) throw neonufft::InputError("Neonufft:: too many points");
+  hipLaunchKernelGGL((band_gather_kernel<T>), dim3(static_cast<unsigned>(nchunks)),
+                     dim3(kRowBlock), 0, stream, jobs, chunks, BandJob<T>{}, 0, std::size_t{0});
+  check_launch("band_gather_kernel");
+}
+
+template <typename T>
+void launch_band_spread(int ns, const BandJob<T>* jobs, const uint32_t* band_job,
+                        const BandJob<T>& single, int njobs, std::size_t total_bands,
+                        std::size_t lds_bytes, bool accumulate, bool fft, const EsParams& es,
+                        hipStream_t stream) {
   if (total_bands == 0) return;
   if (fft && accumulate) throw neonufft::InternalError("fused band FFT cannot accumulate");
   if (total_bands > 0x7fffffffu) throw neonufft::InputError("Neonufft:: too many row bands");
 #define NNF_BAND_CASE(NSV)                                                                    \
   case NSV:                                                                                  \
-    band_spread_ns<T, NSV>(jobs, single, njobs, total_bands, lds_bytes, accumulate, fft, es,   \
-                           stream);                                                          \
+    band_spread_ns<T, NSV>(jobs, band_job, single, njobs, total_bands, lds_bytes, accumulate,  \
+                           fft, es, stream);                                                 \
     break;
   switch (ns) {
     NNF_BAND_CASE(2)
@@ -574,11 +616,13 @@ std::size_t band_lds_bytes(int rows, int pitch, std::size_t acc_bytes) {
                                       PointRec<T, 2>*, uint32_t*, uint16_t*, hipStream_t);      \
   template void launch_row_block_count<T>(const PointRec<T, 2>*, std::size_t, int, int,          \
                                           uint32_t*, hipStream_t);                               \
+  template void launch_band_gather_chunks<T>(const BandJob<T>*, const GatherChunk*, std::size_t, \
+                                             hipStream_t);                                        \
   template void launch_band_gather<T>(const BandJob<T>*, const BandJob<T>&, int, std::size_t,   \
                                       hipStream_t);                                             \
-  template void launch_band_spread<T>(int, const BandJob<T>*, const BandJob<T>&, int,           \
-                                      std::size_t, std::size_t, bool, bool, const EsParams&,    \
-                                      hipStream_t);
+  template void launch_band_spread<T>(int, const BandJob<T>*, const uint32_t*, const BandJob<T>&, \
+                                      int, std::size_t, std::size_t, bool, bool,                 \
+                                      const EsParams&, hipStream_t);
 
 NNF_INST_BAND(float)
 NNF_INST_BAND(double)

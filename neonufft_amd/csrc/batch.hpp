@@ -261,6 +261,23 @@ public:
                                      row_host_[t][static_cast<std::size_t>(g.n[1]) + 1],
                                      tune_.spread_mode == kSpreadAuto);
         }
+        if (bands_on_) {
+          // the transform of every band of the launch
+          band_job_host_.resize(total_bands_);
+          for (int t = 0; t < count_; ++t) {
+            const Group& g = groups_[items_[t].group];
+            const std::size_t nb = static_cast<std::size_t>((g.n[1] + g.band.rows - 1) / g.band.rows);
+            std::fill_n(band_job_host_.begin() + static_cast<std::ptrdiff_t>(items_[t].band_off), nb,
+                        static_cast<uint32_t>(t));
+          }
+          band_job_.ensure(total_bands_ ? total_bands_ : 1, alloc_);
+          if (total_bands_) {
+            check_hip(hipMemcpyAsync(band_job_.data(), band_job_host_.data(),
+                                     total_bands_ * sizeof(uint32_t), hipMemcpyHostToDevice, stream_),
+                      "hipMemcpyAsync(band jobs)");
+          }
+          build_gather_chunks(num_nu);
+        }
       }
     }
     for (int t = 0; t < count_; ++t) {
@@ -410,9 +427,9 @@ private:
       if (bands_on_) {
         if constexpr (DIM == 2) {
           launch_clear_bounds(bounds_.data(), bounds_.size(), s);
-          launch_band_gather<T>(bjobs_.data(), BandJob<T>{}, count_, total_points_, s);
-          launch_band_spread<T>(ks_.ns, bjobs_.data(), BandJob<T>{}, count_, total_bands_,
-                                band_lds_, false, fused_, ks_.es, s);
+          launch_band_gather_chunks<T>(bjobs_.data(), gather_chunks_.data(), n_gather_chunks_, s);
+          launch_band_spread<T>(ks_.ns, bjobs_.data(), band_job_.data(), BandJob<T>{}, count_,
+                                total_bands_, band_lds_, false, fused_, ks_.es, s);
         }
       } else {
         launch_spread_batch<T, DIM>(ks_.ns, sjobs_.data(), subs_.data(), nsub_, max_lds_, ks_.es,
@@ -509,6 +526,44 @@ private:
     if (!graph_ok_[k]) graph_failed_[k] = true;
   }
 
+  // Gather workgroups: all chunks of a transform at one residue mod 8 of the
+  // workgroup index (one XCD), transforms placed largest first on the XCD
+  // with the fewest points so far, so each transform's random strength reads
+  // stay in one L2 and the XCDs get about equal shares.
+  void build_gather_chunks(const IntType* num_nu) {
+    constexpr int kXcds = 8;
+    std::vector<int> order(static_cast<std::size_t>(count_));
+    for (int t = 0; t < count_; ++t) order[static_cast<std::size_t>(t)] = t;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return num_nu[a] > num_nu[b]; });
+    std::vector<std::vector<GatherChunk>> lists(kXcds);
+    std::size_t load[kXcds] = {};
+    for (int t : order) {
+      const int x = static_cast<int>(std::min_element(load, load + kXcds) - load);
+      const std::size_t m = static_cast<std::size_t>(num_nu[t]);
+      for (std::size_t first = 0; first < m; first += kGatherChunk) {
+        lists[static_cast<std::size_t>(x)].push_back(
+            GatherChunk{static_cast<uint32_t>(t), static_cast<uint32_t>(first)});
+      }
+      load[x] += m;
+    }
+    std::size_t len = 0;
+    for (const auto& l : lists) len = std::max(len, l.size());
+    gather_chunks_host_.assign(len * kXcds, GatherChunk{0xffffffffu, 0u});
+    for (int x = 0; x < kXcds; ++x) {
+      const auto& l = lists[static_cast<std::size_t>(x)];
+      for (std::size_t k = 0; k < l.size(); ++k) gather_chunks_host_[k * kXcds + x] = l[k];
+    }
+    n_gather_chunks_ = gather_chunks_host_.size();
+    gather_chunks_.ensure(n_gather_chunks_ ? n_gather_chunks_ : 1, alloc_);
+    if (n_gather_chunks_) {
+      check_hip(hipMemcpyAsync(gather_chunks_.data(), gather_chunks_host_.data(),
+                               n_gather_chunks_ * sizeof(GatherChunk), hipMemcpyHostToDevice,
+                               stream_),
+                "hipMemcpyAsync(gather chunks)");
+    }
+  }
+
   void drop_graphs() {
     for (int k = 0; k < 2; ++k) {
       if (exec_[k]) (void)hipGraphExecDestroy(exec_[k]);
@@ -545,6 +600,11 @@ private:
   std::size_t total_bands_ = 0, band_lds_ = 0;
   std::vector<RowView<T>> rows_;  // row-sorted views (row bands)
   DeviceBuffer<Cplx<T>> sorted_;  // strengths of all transforms in view order
+  DeviceBuffer<uint32_t> band_job_;      // transform of every row band (row bands)
+  std::vector<uint32_t> band_job_host_;  // its host copy (source of the upload)
+  DeviceBuffer<GatherChunk> gather_chunks_;      // gather workgroup ranges (row bands)
+  std::vector<GatherChunk> gather_chunks_host_;  // their host copy
+  std::size_t n_gather_chunks_ = 0;
   DeviceBuffer<BandJob<T>> bjobs_;
   std::vector<BandJob<T>> host_bjobs_;
   TableUpload up_bjobs_;

@@ -230,13 +230,31 @@ inline std::size_t band_bound_words(int n1, std::size_t tbytes) {
 template <typename T>
 void launch_band_gather(const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
                         std::size_t total_points, hipStream_t stream);
+// Points per workgroup of the gather.
+constexpr int kGatherChunk = 2048;
+// One gather workgroup's range: points [first, first + kGatherChunk) of job
+// `job` (job = 0xffffffff: no work).
+struct GatherChunk {
+  uint32_t job;
+  uint32_t first;
+};
+// The same gather over the device table `jobs`, one workgroup per entry of
+// `chunks`. Workgroup i runs on XCD i mod 8, so a table that places all of a
+// job's chunks at one residue mod 8 keeps that job's random strength reads in
+// one XCD's L2.
+template <typename T>
+void launch_band_gather_chunks(const BandJob<T>* jobs, const GatherChunk* chunks,
+                               std::size_t nchunks, hipStream_t stream);
 // Forms (or adds to) every job's fine grid from its gathered strengths, one
 // workgroup per band; with `fft`, transforms each band along x in LDS and
 // writes the retained modes to job.h instead (no grid, no accumulate).
+// `band_job` (optional, with `jobs`): the job of every band of the launch, so
+// a workgroup finds its job with one load instead of a search over `jobs`.
 template <typename T>
-void launch_band_spread(int ns, const BandJob<T>* jobs, const BandJob<T>& single, int njobs,
-                        std::size_t total_bands, std::size_t lds_bytes, bool accumulate, bool fft,
-                        const EsParams& es, hipStream_t stream);
+void launch_band_spread(int ns, const BandJob<T>* jobs, const uint32_t* band_job,
+                        const BandJob<T>& single, int njobs, std::size_t total_bands,
+                        std::size_t lds_bytes, bool accumulate, bool fft, const EsParams& es,
+                        hipStream_t stream);
 std::size_t band_lds_bytes(int rows, int pitch, std::size_t acc_bytes);
 // LDS row of a band: cells x in [-kBandLeft, pitch - kBandLeft) (the cells
 // outside [0, n0) catch kernel runs that cross the row's ends and are folded

@@ -321,7 +321,8 @@ public:
     if constexpr (DIM == 2) {
       if (bands_on_) {
         const int nb = static_cast<int>((n_[1] + band_rows_ - 1) / band_rows_);
-        launch_band_spread<T>(ks_.ns, nullptr, band_job(nullptr), 1, static_cast<std::size_t>(nb),
+        launch_band_spread<T>(ks_.ns, nullptr, nullptr, band_job(nullptr), 1,
+                              static_cast<std::size_t>(nb),
                               band_lds_bytes(band_rows_, band_.pitch, sizeof(T)), accumulate,
                               false, ks_.es, stream_);
         return;
@@ -352,7 +353,8 @@ public:
         launch_clear_bounds(bounds_.data(), bounds_.size(), stream_);
         launch_band_gather<T>(nullptr, band_job(in), 1, pts_.size(), stream_);
         const int nb = static_cast<int>((n_[1] + band_rows_ - 1) / band_rows_);
-        launch_band_spread<T>(ks_.ns, nullptr, band_job(nullptr), 1, static_cast<std::size_t>(nb),
+        launch_band_spread<T>(ks_.ns, nullptr, nullptr, band_job(nullptr), 1,
+                              static_cast<std::size_t>(nb),
                               band_lds_bytes(band_rows_, band_.pitch, sizeof(T)), false, true,
                               ks_.es, stream_);
         forward_from_h(fs, out, out_strides);
