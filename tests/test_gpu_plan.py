@@ -444,6 +444,65 @@ def test_axis_fft_odd_modes(prec):
         assert d < (1e-5 if prec == "f32" else 1e-12), (t, d)
 
 
+Y_LDS_CASES = [
+    # (modes, upsampfac, fine grid): x' groups of R = 8 / 4 / 16 lines, the
+    # last group partial for 43 x modes; odd retained y counts
+    ((43, 683), 1.5, (64, 1024)),
+    ((32, 1024), 2.0, (64, 2048)),
+    ((43, 171), 1.5, (64, 256)),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("order", [nnf.ModeOrder.CMCL, nnf.ModeOrder.FFT])
+@pytest.mark.parametrize("case", Y_LDS_CASES, ids=[f"m{c[0][0]}x{c[0][1]}" for c in Y_LDS_CASES])
+def test_type1_y_pass_in_lds(case, order, prec):
+    """2D type 1 on fine grids with power-of-two n1: the y pass and the
+    deconvolution run in LDS after the band kernel's x-FFT (y_out2_kernel),
+    over full and partial groups of x modes, with caller strides in both
+    orders (x fastest, y fastest): against the FP64 direct sum, the FP64
+    restatement and the plan whose bands write the fine grid (rocFFT passes
+    and the transposing deconvolution)."""
+    modes, sigma, grid = case
+    m = 4000
+    tol = 1e-5 if prec == "f32" else 1e-9
+    sign = -1
+    rng = np.random.default_rng(hash(case) & 0xFFFF)
+    x = points(rng, 2, m, prec)
+    c = cplx(rng, m, prec)
+    loc = [dev(v, RDT[prec]) for v in x]
+    cin = dev(c, CDT[prec])
+    opt = nnf.Options(tol=tol, upsampfac=sigma, order=order)
+    nm = int(np.prod(modes))
+    outs = {}
+    for mode in ("bands", "bands_grid"):
+        plan = nnf.Plan(opt, sign, m, loc, modes, spread_mode=mode)
+        assert tuple(plan.info()["grid"]) == grid
+        assert plan.info()["spread_mode"] == mode
+        out = torch.zeros(nm, dtype=CDT[prec], device="cuda")
+        plan.transform_type_1(cin, out)
+        out_t = torch.zeros(nm, dtype=CDT[prec], device="cuda")
+        plan.transform_type_1(cin, out_t, [modes[1], 1])  # y fastest
+        torch.cuda.synchronize()
+        got = out.cpu().numpy().astype(np.complex128)
+        got_t = out_t.cpu().numpy().astype(np.complex128).reshape(modes[0], modes[1]).T.ravel()
+        outs[mode] = (got, got_t)
+    xs = [v.astype(np.float64) for v in x]
+    c64 = c.astype(np.complex128)
+    offt = order == nnf.ModeOrder.FFT
+    idx = sample_modes(modes, rng)
+    ref = O.direct_type1(xs, c64, O.mode_vectors(modes, idx, offt), sign)
+    alg = O.nufft1(xs, c64, modes, sign, tol, upsampfac=sigma, order_fft=offt)
+    amp = amplification(modes, tol, sigma)
+    same = 1e-12 if prec == "f64" else 5e-6
+    for mode, (got, got_t) in outs.items():
+        assert O.rel_l2(got[idx], ref) < bound(prec, O.rel_l2(alg[idx], ref), tol, amp), mode
+        assert O.rel_l2(got, alg) < alg_bound(prec, tol, amp), mode
+        assert O.rel_l2(got_t, got) < same, mode
+    assert O.rel_l2(outs["bands"][0], outs["bands_grid"][0]) < same
+
+
 ROW_CASES = [
     # (dim, modes, M, tol, sign): fine grids with a power-of-two x extent, so
     # type 1 joins the tile reduction to the x-FFT and type 2 the zero padding
