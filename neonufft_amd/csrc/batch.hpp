@@ -476,7 +476,13 @@ private:
           g.fft.execute(grids_.data() + g.offset);
         } else {
           g.pfft.set_stream(s);
-          g.pfft.backward(nullptr, ins, st, fs, order_fft(), grids_.data() + g.offset, s);
+          if (DIM == 2 && fused_ && g.pfft.y_lds_rows() > 0) {
+            // twiddles up to the largest n1 exist for the fused type 1
+            g.pfft.backward_lds(tw_.data(), nullptr, ins, st, fs, order_fft(),
+                                grids_.data() + g.offset, s);
+          } else {
+            g.pfft.backward(nullptr, ins, st, fs, order_fft(), grids_.data() + g.offset, s);
+          }
         }
       }
       launch_interp_batch<T, DIM>(ks_.ns, ijobs_.data(), subs_.data(), nsub_, max_lds_, ks_.es, s);

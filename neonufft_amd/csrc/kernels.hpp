@@ -326,17 +326,22 @@ inline int axis_fft_rows(IntType n, std::size_t tbytes, std::size_t budget) {
   }
   return 0;
 }
-// 2D type 1 after the x pass: h holds, per transform b (at b * member_stride),
-// N0 retained x' lines of n1 cells, y fastest. One workgroup transforms
-// `rows` lines (a power of two) in LDS along y and writes the retained y'
-// deconvolved into the transform's modes (`single`, or the device table
-// `outs`, in the caller's order and strides).
+// The y pass of 2D plans in LDS. h holds, per transform b (at b *
+// member_stride), N0 retained x' lines of n1 cells, y fastest. One workgroup
+// transforms `rows` lines (a power of two) along y.
+//   type 1 (launch_y_out2): h -> y-FFT -> the retained y' deconvolved into the
+//     transform's modes (`single`, or the device table `outs`);
+//   type 2 (launch_y_in2): the transform's modes (`single_in` or `ins`)
+//     deconvolved into zero-padded y lines -> y-FFT -> h.
+// Modes in the caller's order and strides.
 template <typename T>
-struct YOut2Job {
-  const Cplx<T>* h;
+struct Y2Job {
+  Cplx<T>* h;
   long long member_stride;
   Cplx<T>* single;
   Cplx<T>* const* outs;
+  const Cplx<T>* single_in;
+  const Cplx<T>* const* ins;
   const T* fs0;
   const T* fs1;
   long long st0, st1;
@@ -345,7 +350,9 @@ struct YOut2Job {
   int count, N0, N1, n1, log2n1, rows, log2rows;
 };
 template <typename T>
-void launch_y_out2(const YOut2Job<T>& job, hipStream_t stream);
+void launch_y_out2(const Y2Job<T>& job, hipStream_t stream);
+template <typename T>
+void launch_y_in2(const Y2Job<T>& job, hipStream_t stream);
 
 template <typename T>
 void launch_axis_fft_type1(const AxisFftJob<T>& job, hipStream_t stream);

@@ -395,6 +395,8 @@ public:
       if (row_rows_ > 0) {
         if (axis_rows_y_ > 0) {
           launch_axis_fft_type2<T>(axis_job(nullptr, in, in_strides), stream_);
+        } else if (DIM == 2 && pfft_.y_lds_rows() > 0) {
+          pfft_.backward_to_h_lds(tw_.data(), in, nullptr, in_strides, fs, order_fft(), stream_);
         } else {
           pfft_.backward_to_h(in, nullptr, in_strides, fs, order_fft(), stream_);
         }

@@ -104,27 +104,29 @@ public:
   // instead of the rocFFT pass plus the transposing deconvolution.
   void forward_from_h_lds(const Cplx<T>* tw, T* const* fs, int order_fft, Cplx<T>* single,
                           Cplx<T>* const* outs, const IntType* st, hipStream_t s) {
-    YOut2Job<T> j{};
-    j.h = h_.data();
-    j.member_stride = N_[0] * n_[1];
+    Y2Job<T> j = y_job(tw, fs, order_fft, st);
     j.single = single;
     j.outs = outs;
-    j.fs0 = fs[0];
-    j.fs1 = fs[1];
-    j.st0 = st[0];
-    j.st1 = st[1];
-    j.order_fft = order_fft;
-    j.tw = tw;
-    j.count = count_;
-    j.N0 = static_cast<int>(N_[0]);
-    j.N1 = static_cast<int>(N_[1]);
-    j.n1 = static_cast<int>(n_[1]);
-    j.rows = y_rows_;
-    for (j.log2n1 = 0; (1LL << j.log2n1) < n_[1]; ++j.log2n1) {
-    }
-    for (j.log2rows = 0; (1 << j.log2rows) < y_rows_; ++j.log2rows) {
-    }
     launch_y_out2<T>(j, s);
+  }
+
+  // 2D type 2 up to h() with the y pass in LDS (same conditions): the
+  // deconvolved, zero-padded and y-transformed modes in one read of the modes
+  // and one write of h.
+  void backward_to_h_lds(const Cplx<T>* tw, const Cplx<T>* single, const Cplx<T>* const* ins,
+                         const IntType* st, T* const* fs, int order_fft, hipStream_t s) {
+    Y2Job<T> j = y_job(tw, fs, order_fft, st);
+    j.single_in = single;
+    j.ins = ins;
+    launch_y_in2<T>(j, s);
+  }
+
+  // 2D type 2 into the fine grids with the y pass in LDS (y_lds_rows() > 0).
+  void backward_lds(const Cplx<T>* tw, const Cplx<T>* single, const Cplx<T>* const* ins,
+                    const IntType* st, T* const* fs, int order_fft, Cplx<T>* grids,
+                    hipStream_t s) {
+    backward_to_h_lds(tw, single, ins, st, fs, order_fft, s);
+    expand_from_h(grids, s);
   }
 
   // Type 2 up to h(): deconvolution, zero padding and the passes after x; the
@@ -174,9 +176,17 @@ public:
   // `ins`) -> deconvolved, zero-padded and transformed fine grids.
   void backward(const Cplx<T>* single, const Cplx<T>* const* ins, const IntType* st,
                 T* const* fs, int order_fft, Cplx<T>* grids, hipStream_t s) {
+    backward_to_h(single, ins, st, fs, order_fft, s);
+    expand_from_h(grids, s);
+  }
+
+private:
+  // After the y-FFT of h: 2D deconvolves from h; 3D transposes to K, runs the
+  // z-FFT and deconvolves.
+  // h -> zero-padded x rows of the fine grids -> x-FFT.
+  void expand_from_h(Cplx<T>* grids, hipStream_t s) {
     const long long n0 = n_[0], n1 = n_[1], n2 = n_[2], N0 = N_[0];
     const long long gstride = n0 * n1 * n2;
-    backward_to_h(single, ins, st, fs, order_fft, s);
     launch_expand_transpose<T>(h_.data(), grids, static_cast<int>(n1), static_cast<int>(n0),
                                static_cast<int>(N0), n1, n0,
                                static_cast<std::size_t>(count_ * n2), static_cast<int>(n2),
@@ -184,9 +194,28 @@ public:
     px_.execute(grids);
   }
 
-private:
-  // After the y-FFT of h: 2D deconvolves from h; 3D transposes to K, runs the
-  // z-FFT and deconvolves.
+  Y2Job<T> y_job(const Cplx<T>* tw, T* const* fs, int order_fft, const IntType* st) const {
+    Y2Job<T> j{};
+    j.h = h_.data();
+    j.member_stride = N_[0] * n_[1];
+    j.fs0 = fs[0];
+    j.fs1 = fs[1];
+    j.st0 = st[0];
+    j.st1 = st[1];
+    j.order_fft = order_fft;
+    j.tw = tw;
+    j.count = count_;
+    j.N0 = static_cast<int>(N_[0]);
+    j.N1 = static_cast<int>(N_[1]);
+    j.n1 = static_cast<int>(n_[1]);
+    j.rows = y_rows_;
+    for (j.log2n1 = 0; (1LL << j.log2n1) < n_[1]; ++j.log2n1) {
+    }
+    for (j.log2rows = 0; (1 << j.log2rows) < y_rows_; ++j.log2rows) {
+    }
+    return j;
+  }
+
   void finish_forward(T* const* fs, int order_fft, Cplx<T>* single, Cplx<T>* const* outs,
                       const IntType* st, hipStream_t s) {
     const long long n1 = n_[1], n2 = n_[2], N0 = N_[0], N1 = N_[1], N2 = N_[2];

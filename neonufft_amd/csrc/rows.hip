@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(kRowThreads<T>) z_out_kernel(AxisFftJob<T> job
 // x'). Neighbouring x' groups of a transform map to one XCD, so the pieces
 // of output lines they write (rows x 8 bytes each) meet in one L2.
 template <typename T>
-__global__ void __launch_bounds__(kRowThreads<T>) y_out2_kernel(YOut2Job<T> job) {
+__global__ void __launch_bounds__(kRowThreads<T>) y_out2_kernel(Y2Job<T> job) {
   constexpr int NT = kRowThreads<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int R = job.rows, n1 = job.n1, N0 = job.N0, N1 = job.N1;
@@ -445,6 +445,57 @@ __global__ void __launch_bounds__(kRowThreads<T>) y_out2_kernel(YOut2Job<T> job)
     out[static_cast<long long>(order_index(kx, N0, job.order_fft)) * job.st0 +
         static_cast<long long>(order_index(ky, N1, job.order_fft)) * job.st1] =
         Cplx<T>{fre[pos] * fac, fim[pos] * fac};
+  }
+}
+
+// 2D type 2: transform b's modes of x' = x0 .. x0 + rk, deconvolved, at their
+// y positions of zero-padded lines; the transformed lines go to h. Same
+// workgroup map as y_out2_kernel.
+template <typename T>
+__global__ void __launch_bounds__(kRowThreads<T>) y_in2_kernel(Y2Job<T> job) {
+  constexpr int NT = kRowThreads<T>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int R = job.rows, n1 = job.n1, N0 = job.N0, N1 = job.N1;
+  const int pitch = n1 + 8;
+  T* fre = reinterpret_cast<T*>(smem_raw);
+  T* fim = fre + R * pitch;
+  const int groups = (N0 + R - 1) / R;
+  const int lb = static_cast<int>(
+      xcd_remap(blockIdx.x, static_cast<uint32_t>(job.count) * static_cast<uint32_t>(groups)));
+  const int b = lb / groups;
+  const int x0 = (lb - b * groups) * R;
+  const int rk = min(R, N0 - x0);
+  for (int i = threadIdx.x; i < 2 * R * pitch; i += NT) fre[i] = T(0);
+  __syncthreads();
+  const int lr = job.log2rows;
+  const Cplx<T>* __restrict__ in = job.ins ? job.ins[b] : job.single_in;
+  // items (r fastest, then y'); rows r >= rk repeat row rk - 1 and are dropped
+  batched_copy<NT>(
+      N1 << lr,
+      [&](int i) {
+        const int r = min(i & (R - 1), rk - 1), lp = i >> lr;
+        const int kx = ret_mode(x0 + r, N0), ky = ret_mode(lp, N1);
+        const T fac = job.fs0[kx < 0 ? -kx : kx] * job.fs1[ky < 0 ? -ky : ky];
+        const Cplx<T> a = in[static_cast<long long>(order_index(kx, N0, job.order_fft)) * job.st0 +
+                             static_cast<long long>(order_index(ky, N1, job.order_fft)) * job.st1];
+        return Cplx<T>{a.x * fac, a.y * fac};
+      },
+      [&](int i, const Cplx<T>& v) {
+        const int r = i & (R - 1), lp = i >> lr;
+        if (r >= rk) return;
+        const int o = r * pitch + ret_fine(lp, n1, N1);
+        fre[o] = v.x;
+        fim[o] = v.y;
+      });
+  __syncthreads();
+  const int l1 = job.log2n1;
+  lds_fft_x<T, NT, false>(fre, fim, rk, pitch, l1, T(1), job.tw);
+  const int sh = 32 - l1;
+  Cplx<T>* __restrict__ dst = job.h + b * job.member_stride + static_cast<long long>(x0) * n1;
+  for (int i = threadIdx.x; i < (rk << l1); i += NT) {
+    const int r = i >> l1, y = i & (n1 - 1);
+    const int pos = r * pitch + static_cast<int>(__brev(static_cast<unsigned>(y)) >> sh);
+    dst[static_cast<long long>(r) * n1 + y] = Cplx<T>{fre[pos], fim[pos]};
   }
 }
 
@@ -601,7 +652,7 @@ void launch_axis_fft_type2(const AxisFftJob<T>& job, hipStream_t stream) {
 }
 
 template <typename T>
-void launch_y_out2(const YOut2Job<T>& job, hipStream_t stream) {
+void launch_y_out2(const Y2Job<T>& job, hipStream_t stream) {
   const std::size_t blocks =
       static_cast<std::size_t>(job.count) * ((job.N0 + job.rows - 1) / job.rows);
   if (blocks == 0 || job.N1 == 0) return;
@@ -612,8 +663,22 @@ void launch_y_out2(const YOut2Job<T>& job, hipStream_t stream) {
   check_launch("y_out2_kernel");
 }
 
-template void launch_y_out2<float>(const YOut2Job<float>&, hipStream_t);
-template void launch_y_out2<double>(const YOut2Job<double>&, hipStream_t);
+template <typename T>
+void launch_y_in2(const Y2Job<T>& job, hipStream_t stream) {
+  const std::size_t blocks =
+      static_cast<std::size_t>(job.count) * ((job.N0 + job.rows - 1) / job.rows);
+  if (blocks == 0) return;
+  if (blocks > 0xFFFFFFFFull) throw neonufft::InputError("Neonufft:: y pass too large");
+  hipLaunchKernelGGL((y_in2_kernel<T>), dim3(static_cast<unsigned>(blocks)),
+                     dim3(kRowThreads<T>), row_fft_lds_bytes(job.rows, job.n1 + 8, sizeof(T), 0),
+                     stream, job);
+  check_launch("y_in2_kernel");
+}
+
+template void launch_y_out2<float>(const Y2Job<float>&, hipStream_t);
+template void launch_y_out2<double>(const Y2Job<double>&, hipStream_t);
+template void launch_y_in2<float>(const Y2Job<float>&, hipStream_t);
+template void launch_y_in2<double>(const Y2Job<double>&, hipStream_t);
 template void launch_axis_fft_type1<float>(const AxisFftJob<float>&, hipStream_t);
 template void launch_axis_fft_type1<double>(const AxisFftJob<double>&, hipStream_t);
 template void launch_axis_fft_type2<float>(const AxisFftJob<float>&, hipStream_t);

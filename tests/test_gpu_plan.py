@@ -457,13 +457,14 @@ Y_LDS_CASES = [
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("order", [nnf.ModeOrder.CMCL, nnf.ModeOrder.FFT])
 @pytest.mark.parametrize("case", Y_LDS_CASES, ids=[f"m{c[0][0]}x{c[0][1]}" for c in Y_LDS_CASES])
-def test_type1_y_pass_in_lds(case, order, prec):
-    """2D type 1 on fine grids with power-of-two n1: the y pass and the
-    deconvolution run in LDS after the band kernel's x-FFT (y_out2_kernel),
-    over full and partial groups of x modes, with caller strides in both
-    orders (x fastest, y fastest): against the FP64 direct sum, the FP64
-    restatement and the plan whose bands write the fine grid (rocFFT passes
-    and the transposing deconvolution)."""
+def test_y_pass_in_lds(case, order, prec):
+    """2D plans on fine grids with power-of-two n1: the y pass and the
+    deconvolution run in LDS (type 1: y_out2_kernel after the band kernel's
+    x-FFT; type 2: y_in2_kernel before the row kernel's x-FFT), over full and
+    partial groups of x modes, with caller strides in both orders (x
+    fastest, y fastest): against the FP64 direct sum, the FP64 restatement
+    and plans on rocFFT passes with transposes (type 1: bands writing the
+    fine grid; type 2: subgrid tiles without the row kernels)."""
     modes, sigma, grid = case
     m = 4000
     tol = 1e-5 if prec == "f32" else 1e-9
@@ -501,6 +502,29 @@ def test_type1_y_pass_in_lds(case, order, prec):
         assert O.rel_l2(got, alg) < alg_bound(prec, tol, amp), mode
         assert O.rel_l2(got_t, got) < same, mode
     assert O.rel_l2(outs["bands"][0], outs["bands_grid"][0]) < same
+    # type 2: modes in `order`, x fastest, then the same array y fastest
+    f = cplx(rng, nm, prec)
+    f_t = f.reshape(modes[1], modes[0]).T.ravel()
+    res = {}
+    for mode in ("auto", "tiles"):
+        plan = nnf.Plan(opt, sign, m, loc, modes, spread_mode=mode)
+        g = torch.zeros(m, dtype=CDT[prec], device="cuda")
+        plan.transform_type_2(dev(f, CDT[prec]), None, g)
+        g_t = torch.zeros(m, dtype=CDT[prec], device="cuda")
+        plan.transform_type_2(dev(f_t, CDT[prec]), [modes[1], 1], g_t)
+        torch.cuda.synchronize()
+        res[mode] = (g.cpu().numpy().astype(np.complex128), g_t.cpu().numpy().astype(np.complex128))
+    f64 = f.astype(np.complex128)
+    f_cmcl = f64.reshape(modes[1], modes[0])
+    if offt:  # the direct sum takes CMCL order
+        f_cmcl = np.roll(np.roll(f_cmcl, modes[1] // 2, axis=0), modes[0] // 2, axis=1)
+    ref2 = O.direct_type2(xs, f_cmcl.ravel(), modes, sign)
+    alg2 = O.nufft2(xs, f64, modes, sign, tol, upsampfac=sigma, order_fft=offt)
+    for mode, (g, g_t) in res.items():
+        assert O.rel_l2(g, ref2) < bound(prec, O.rel_l2(alg2, ref2), tol, amp), mode
+        assert O.rel_l2(g, alg2) < alg_bound(prec, tol, amp), mode
+        assert O.rel_l2(g_t, g) < same, mode
+    assert O.rel_l2(res["auto"][0], res["tiles"][0]) < same
 
 
 ROW_CASES = [
