@@ -1,0 +1,137 @@
+"""GPU checks at the full size of the 3D workload: 1e7 uniform points in
+[-pi, pi)^3, 128^3 modes, tol 1e-5, upsampling 2, fp32 (fine grid 256^3,
+8192 subproblems, the row kernels and the LDS y / z passes).
+
+- Type 1 and type 2 against the FP64 direct sum on sampled outputs (the zero
+  mode of type 1 is the plain sum of the strengths) and against the FP64
+  restatement of the same algorithm at full size.
+- Linearity of type 1: T1(a c1 + b c2) = a T1(c1) + b T1(c2). Each call
+  scales its fixed-point tiles by its own strengths, so this holds only to
+  the fp32 share of the error.
+- The adjoint identity <T1_s c, f> = <c, T2_-s f> between a type-1 plan of
+  sign s and a type-2 plan of sign -s. Each side differs from the exact
+  bilinear form by the transform's error (~tol relative) projected onto a
+  random vector, i.e. by ~tol |T1 c| |f| / sqrt(modes); the wrong sign makes
+  them differ by their own size, ~|T1 c| |f| / sqrt(modes).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import neonufft_amd as nnf
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+M = 10_000_000
+N = 128
+MODES = (N, N, N)
+TOL = 1e-5
+SIGN = -1
+
+
+def _dev(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+def _host(t):
+    return t.cpu().numpy().astype(np.complex128)
+
+
+def _cplx(rng, n):
+    return (rng.standard_normal(n, dtype=np.float32)
+            + 1j * rng.standard_normal(n, dtype=np.float32)).astype(np.complex64)
+
+
+@pytest.fixture(scope="module")
+def data():
+    rng = np.random.default_rng(20261203)
+    x = [rng.uniform(-np.pi, np.pi, M).astype(np.float32) for _ in range(3)]
+    loc = [_dev(v, torch.float32) for v in x]
+    opt = nnf.Options(tol=TOL, upsampfac=2.0)
+    plan = nnf.Plan(opt, SIGN, M, loc, MODES)
+    info = plan.info()
+    assert tuple(info["grid"]) == (256, 256, 256)
+    return {"rng": rng, "x": x, "xs": [v.astype(np.float64) for v in x], "loc": loc,
+            "opt": opt, "plan": plan}
+
+
+def _bound(alg_err):
+    # fp32 share at sigma = 2: rounding 3e-6 plus the Horner fit 0.04 tol
+    return 1.2 * alg_err + 3e-6 + 0.04 * TOL
+
+
+def test_fullsize_type1(data):
+    rng, plan = data["rng"], data["plan"]
+    c = _cplx(rng, M)
+    out = torch.empty(N ** 3, dtype=torch.complex64, device="cuda")
+    plan.transform_type_1(_dev(c, torch.complex64), out)
+    torch.cuda.synchronize()
+    got = _host(out)
+    c64 = c.astype(np.complex128)
+    # sampled modes, and the zero mode: the plain sum of the strengths
+    zero = (N // 2) + N * (N // 2) + N * N * (N // 2)
+    idx = np.unique(np.concatenate([rng.choice(N ** 3, size=199, replace=False), [zero]]))
+    ref = O.direct_type1(data["xs"], c64, O.mode_vectors(MODES, idx), SIGN)
+    assert abs(ref[np.searchsorted(idx, zero)] - c64.sum()) < 1e-9 * np.abs(c64).sum()
+    alg = O.nufft1(data["xs"], c64, MODES, SIGN, TOL, 2.0)
+    alg_err = O.rel_l2(alg[idx], ref)
+    assert alg_err < 2 * TOL
+    err = O.rel_l2(got[idx], ref)
+    assert err < _bound(alg_err), (err, alg_err)
+    assert O.rel_l2(got, alg) < 3e-6 + 0.04 * TOL
+
+
+def test_fullsize_type2(data):
+    rng, plan = data["rng"], data["plan"]
+    f = _cplx(rng, N ** 3)
+    out = torch.empty(M, dtype=torch.complex64, device="cuda")
+    plan.transform_type_2(_dev(f, torch.complex64), None, out)
+    torch.cuda.synchronize()
+    got = _host(out)
+    f64 = f.astype(np.complex128)
+    idx = np.sort(rng.choice(M, size=200, replace=False))
+    ref = O.direct_type2([v[idx] for v in data["xs"]], f64, MODES, SIGN)
+    alg = O.nufft2(data["xs"], f64, MODES, SIGN, TOL, 2.0)
+    alg_err = O.rel_l2(alg[idx], ref)
+    assert alg_err < 2 * TOL
+    err = O.rel_l2(got[idx], ref)
+    assert err < _bound(alg_err), (err, alg_err)
+    assert O.rel_l2(got, alg) < 3e-6 + 0.04 * TOL
+
+
+def test_fullsize_type1_linear(data):
+    rng, plan = data["rng"], data["plan"]
+    c1, c2 = _cplx(rng, M), _cplx(rng, M)
+    a, b = 0.75 - 0.5j, -1.25 + 2.0j
+    comb = (a * c1.astype(np.complex128) + b * c2.astype(np.complex128)).astype(np.complex64)
+    outs = []
+    for c in (c1, c2, comb):
+        o = torch.empty(N ** 3, dtype=torch.complex64, device="cuda")
+        plan.transform_type_1(_dev(c, torch.complex64), o)
+        outs.append(o)
+    torch.cuda.synchronize()
+    g1, g2, gc = (_host(o) for o in outs)
+    lin = a * g1 + b * g2
+    assert O.rel_l2(gc, lin) < 1e-5
+
+
+def test_fullsize_adjoint(data):
+    rng = data["rng"]
+    plan1 = data["plan"]
+    plan2 = nnf.Plan(data["opt"], -SIGN, M, data["loc"], MODES)
+    c = _cplx(rng, M)
+    f = _cplx(rng, N ** 3)
+    t1 = torch.empty(N ** 3, dtype=torch.complex64, device="cuda")
+    plan1.transform_type_1(_dev(c, torch.complex64), t1)
+    t2 = torch.empty(M, dtype=torch.complex64, device="cuda")
+    plan2.transform_type_2(_dev(f, torch.complex64), None, t2)
+    torch.cuda.synchronize()
+    g1, g2 = _host(t1), _host(t2)
+    f64, c64 = f.astype(np.complex128), c.astype(np.complex128)
+    lhs = np.vdot(f64, g1)   # <T1 c, f> = sum_k T1(c)_k conj(f_k)
+    rhs = np.vdot(g2, c64)   # <c, T2 f> = sum_j c_j conj(T2(f)_j)
+    scale = np.linalg.norm(g1) * np.linalg.norm(f64) / np.sqrt(N ** 3)
+    assert abs(lhs) > 0.05 * scale  # the form itself is of the expected size
+    assert abs(lhs - rhs) < 20 * TOL * scale, (abs(lhs - rhs) / scale)
