@@ -13,6 +13,9 @@
   bilinear form by the transform's error (~tol relative) projected onto a
   random vector, i.e. by ~tol |T1 c| |f| / sqrt(modes); the wrong sign makes
   them differ by their own size, ~|T1 c| |f| / sqrt(modes).
+
+The clustered type-2 workload (2e7 points, 256^3 modes) is checked at its full
+size as well, on sampled points.
 """
 import numpy as np
 import pytest
@@ -135,3 +138,39 @@ def test_fullsize_adjoint(data):
     scale = np.linalg.norm(g1) * np.linalg.norm(f64) / np.sqrt(N ** 3)
     assert abs(lhs) > 0.05 * scale  # the form itself is of the expected size
     assert abs(lhs - rhs) < 20 * TOL * scale, (abs(lhs - rhs) / scale)
+
+
+@pytest.mark.parametrize("tol,tuned", [(1e-4, False), (1e-6, True)], ids=["tol1e-4_default",
+                                                                           "tol1e-6_auto"])
+def test_fullsize_clustered_type2(tol, tuned):
+    """The clustered type-2 workload at full size: 2e7 points in 16 Gaussians
+    (sigma 0.05), 256^3 modes (fine grid 512^3), the default plan at tol 1e-4
+    and the autotuned one at 1e-6; 200 sampled points against the FP64
+    direct sum and the FP64 restatement (type 2 at a point depends on that
+    point alone, so both run on the sample)."""
+    from test_gpu_clustered import clustered
+    from test_gpu_plan import alg_bound, amplification, bound
+
+    m, modes = 20_000_000, (256, 256, 256)
+    rng = np.random.default_rng(0xC3 + int(tuned))
+    x = clustered(rng, m, np.float32)
+    f = _cplx(rng, int(np.prod(modes)))
+    plan = nnf.Plan(nnf.Options(tol=tol), SIGN, m, [_dev(v, torch.float32) for v in x], modes,
+                    bin_size="auto" if tuned else None)
+    assert tuple(plan.info()["grid"]) == (512, 512, 512)
+    out = torch.empty(m, dtype=torch.complex64, device="cuda")
+    plan.transform_type_2(_dev(f, torch.complex64), None, out)
+    torch.cuda.synchronize()
+    idx = np.sort(rng.choice(m, size=200, replace=False))
+    got = out[torch.from_numpy(idx).to("cuda")].cpu().numpy().astype(np.complex128)
+    del plan, out
+    xs = [v[idx].astype(np.float64) for v in x]
+    f64 = f.astype(np.complex128)
+    ref = O.direct_type2(xs, f64, modes, SIGN)
+    alg = O.nufft2(xs, f64, modes, SIGN, tol)
+    alg_err = O.rel_l2(alg, ref)
+    amp = amplification(modes, tol, 2.0)
+    assert alg_err < 20 * tol
+    err = O.rel_l2(got, ref)
+    assert err < bound("f32", alg_err, tol, amp), (err, alg_err)
+    assert O.rel_l2(got, alg) < alg_bound("f32", tol, amp)
