@@ -14,8 +14,9 @@
   random vector, i.e. by ~tol |T1 c| |f| / sqrt(modes); the wrong sign makes
   them differ by their own size, ~|T1 c| |f| / sqrt(modes).
 
-The clustered type-2 workload (2e7 points, 256^3 modes) is checked at its full
-size as well, on sampled points.
+The clustered type-2 workload (2e7 points, 256^3 modes), the type-3 workload
+(2e6 sources, 1e6 targets) and the batched workload (256 uneven 2D type-1
+transforms) are checked at their full sizes as well, on sampled outputs.
 """
 import numpy as np
 import pytest
@@ -174,3 +175,90 @@ def test_fullsize_clustered_type2(tol, tuned):
     err = O.rel_l2(got, ref)
     assert err < bound("f32", alg_err, tol, amp), (err, alg_err)
     assert O.rel_l2(got, alg) < alg_bound("f32", tol, amp)
+
+
+def test_fullsize_type3_mixed():
+    """The type-3 workload at full size: 2e6 sources with x, y in [-1e4, 1e4]
+    and z in [-50, 50], 1e6 targets in [-0.02, 0.02]^2 x [-1, 1], tol 1e-6, the
+    mixed plan (double coordinates and complex128 I/O, FP64 fold and phases,
+    fp32 spreading / FFT / interpolation). 200 sampled targets against the
+    direct sum over the double coordinates and the FP64 restatement with the
+    grids sized from the full point sets' extents, as the plan's are; the
+    fp32 share is held to 5e-6 (measured at this size: ~1.5e-6)."""
+    m, n, tol = 2_000_000, 1_000_000, 1e-6
+    rng = np.random.default_rng(0x7733)
+    x = [rng.uniform(-1e4, 1e4, m), rng.uniform(-1e4, 1e4, m), rng.uniform(-50, 50, m)]
+    s = [rng.uniform(-0.02, 0.02, n), rng.uniform(-0.02, 0.02, n), rng.uniform(-1, 1, n)]
+    c = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    plan = nnf.PlanT3(nnf.Options(tol=tol), SIGN, m, [_dev(v, torch.float64) for v in x], n,
+                      [_dev(v, torch.float64) for v in s], dtype="mixed")
+    assert plan.info()["precision"] == "mixed"
+    out = torch.empty(n, dtype=torch.complex128, device="cuda")
+    plan.add_input(_dev(c, torch.complex128))
+    plan.transform(out)
+    torch.cuda.synchronize()
+    got_all = out.cpu().numpy()
+    del plan, out
+    tidx = np.sort(rng.choice(n, size=200, replace=False))
+    st = [v[tidx] for v in s]
+    ref = O.direct_type3(x, c, st, SIGN)
+    bnd = ([float(v.min()) for v in x], [float(v.max()) for v in x],
+           [float(v.min()) for v in s], [float(v.max()) for v in s])
+    alg = O.nufft3(x, c, st, SIGN, tol, bounds=bnd)
+    alg_err = O.rel_l2(alg, ref)
+    assert alg_err < 20 * tol
+    got = got_all[tidx]
+    err = O.rel_l2(got, ref)
+    assert err < 1.2 * alg_err + 5e-6, (err, alg_err)
+    assert O.rel_l2(got, alg) < 5e-6
+
+
+def test_fullsize_batched_2d():
+    """The batched workload at full size: 256 2D type-1 transforms, point
+    counts floor(10^U[4, 6]), modes {256, 512, 1024}^2, tol 1e-5, one
+    BatchPlan replayed through its HIP graph. Every transform against its plain
+    plan (the same kernels), and the sparsest and densest transforms against
+    the FP64 direct sum on sampled modes and the FP64 restatement."""
+    from test_gpu_plan import alg_bound, amplification, bound
+
+    count, tol = 256, 1e-5
+    rng = np.random.default_rng(0xBA7C)
+    ms = np.floor(10.0 ** rng.uniform(4.0, 6.0, count)).astype(np.int64)
+    nmodes = rng.choice([256, 512, 1024], size=count)
+    xs, cs = [], []
+    for t in range(count):
+        xs.append([rng.uniform(-np.pi, np.pi, int(ms[t])).astype(np.float32) for _ in range(2)])
+        cs.append(_cplx(rng, int(ms[t])))
+    locs = [[_dev(v, torch.float32) for v in x] for x in xs]
+    cin = [_dev(c, torch.complex64) for c in cs]
+    modes = [[int(n), int(n)] for n in nmodes]
+    opt = nnf.Options(tol=tol, upsampfac=2.0)
+    batch = nnf.BatchPlan(opt, SIGN, [int(v) for v in ms], locs, modes, use_graph=True)
+    outs = [torch.empty(int(n) ** 2, dtype=torch.complex64, device="cuda") for n in nmodes]
+    batch.transform_type_1(cin, outs)
+    batch.transform_type_1(cin, outs)  # the graph replay
+    torch.cuda.synchronize()
+    assert batch.info()["graph"]
+    worst = 0.0
+    for t in range(count):
+        plan = nnf.Plan(opt, SIGN, int(ms[t]), locs[t], modes[t])
+        ref_t = torch.empty_like(outs[t])
+        plan.transform_type_1(cin[t], ref_t)
+        torch.cuda.synchronize()
+        worst = max(worst, float(torch.linalg.vector_norm(outs[t] - ref_t)
+                                 / torch.linalg.vector_norm(ref_t)))
+        del plan
+    assert worst < 5e-6, worst
+    for t in (int(np.argmin(ms)), int(np.argmax(ms))):
+        md = tuple(modes[t])
+        x64 = [v.astype(np.float64) for v in xs[t]]
+        c64 = cs[t].astype(np.complex128)
+        idx = np.sort(rng.choice(md[0] * md[1], size=200, replace=False))
+        ref = O.direct_type1(x64, c64, O.mode_vectors(md, idx), SIGN)
+        alg = O.nufft1(x64, c64, md, SIGN, tol, 2.0)
+        alg_err = O.rel_l2(alg[idx], ref)
+        amp = amplification(md, tol, 2.0)
+        got = outs[t].cpu().numpy().astype(np.complex128)
+        assert alg_err < 20 * tol
+        assert O.rel_l2(got[idx], ref) < bound("f32", alg_err, tol, amp), t
+        assert O.rel_l2(got, alg) < alg_bound("f32", tol, amp), t
