@@ -386,7 +386,7 @@ def cpu_leg_type2(x, f, out_gpu, modes, tol, sample, nacc):
 
 
 def clustered_leg(args):
-    """cfg3-style workload: 3D type 2 on 2e7 clustered points, 256^3 modes,
+    """Clustered workload: 3D type 2 on 2e7 clustered points, 256^3 modes,
     tol 1e-4 and 1e-6, tiles chosen by the autotuner."""
     import numpy as np
     import torch
@@ -478,7 +478,7 @@ def run_clustered(args):
 
 
 def type3_leg(args):
-    """cfg4-style workload: 3D type 3, 2e6 sources, 1e6 targets, tol 1e-6, in
+    """Type-3 workload: 3D type 3, 2e6 sources, 1e6 targets, tol 1e-6, in
     the mixed, fp32 and fp64 plans."""
     import numpy as np
     import torch

@@ -73,7 +73,7 @@ def test_invalid_precision_and_dim_are_input_errors():
 
 
 def test_t3_grid_memory_size_matches_sizing(orc):
-    # cfg4 shape: x,y in [-1e4,1e4], z in [-50,50]; s_xy in [-0.02,0.02], s_z in [-1,1]
+    # the wide type-3 layout: x,y in [-1e4,1e4], z in [-50,50]; s_xy in [-0.02,0.02], s_z in [-1,1]
     opt = nnf.Options(tol=1e-6)
     size = nnf.PlanT3.grid_memory_size(opt, [-1e4, -1e4, -50], [1e4, 1e4, 50],
                                        [-0.02, -0.02, -1], [0.02, 0.02, 1], dtype="float32")

@@ -145,7 +145,7 @@ def test_next235even_brute_force():
     assert O.next235even(524) == 540 and O.next235even(144) == 144
 
 
-def test_t3_grid_info_cfg4():
+def test_t3_grid_info_wide_layout():
     ns, _ = O.kernel_params(1e-6, 2.0)
     gi = O.GridInfoT3.make(ns, 2.0, 0.1, [-1e4, -1e4, -50], [1e4, 1e4, 50],
                            [-0.02, -0.02, -1], [0.02, 0.02, 1])
