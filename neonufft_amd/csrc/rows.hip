@@ -409,6 +409,7 @@ __global__ void __launch_bounds__(kRowThreads<T>) z_out_kernel(AxisFftJob<T> job
 template <typename T>
 __global__ void __launch_bounds__(kRowThreads<T>) y_out2_kernel(Y2Job<T> job) {
   constexpr int NT = kRowThreads<T>;
+  static_assert(NT % 16 == 0, "workgroup rows (at most 16) must divide the thread count");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int R = job.rows, n1 = job.n1, N0 = job.N0, N1 = job.N1;
   const int pitch = n1 + 8;
@@ -435,16 +436,22 @@ __global__ void __launch_bounds__(kRowThreads<T>) y_out2_kernel(Y2Job<T> job) {
   const int sh = 32 - l1;
   const int lr = job.log2rows;
   Cplx<T>* __restrict__ out = job.outs ? job.outs[b] : job.single;
-  for (int i = threadIdx.x; i < (N1 << lr); i += NT) {
-    const int r = i & (R - 1), lp = i >> lr;
-    if (r >= rk) continue;
-    const int kx = ret_mode(x0 + r, N0), ky = ret_mode(lp, N1);
-    const T fac = job.fs0[kx < 0 ? -kx : kx] * job.fs1[ky < 0 ? -ky : ky];
-    const int f = ret_fine(lp, n1, N1);
-    const int pos = r * pitch + static_cast<int>(__brev(static_cast<unsigned>(f)) >> sh);
-    out[static_cast<long long>(order_index(kx, N0, job.order_fft)) * job.st0 +
-        static_cast<long long>(order_index(ky, N1, job.order_fft)) * job.st1] =
-        Cplx<T>{fre[pos] * fac, fim[pos] * fac};
+  // a lane keeps its line r (NT is a multiple of R) and steps through y'
+  const int r = static_cast<int>(threadIdx.x) & (R - 1);
+  if (r < rk) {
+    const int kx = ret_mode(x0 + r, N0);
+    const T fx = job.fs0[kx < 0 ? -kx : kx];
+    Cplx<T>* __restrict__ orow =
+        out + static_cast<long long>(order_index(kx, N0, job.order_fft)) * job.st0;
+    const int rp = r * pitch;
+    for (int lp = static_cast<int>(threadIdx.x) >> lr; lp < N1; lp += NT >> lr) {
+      const int ky = ret_mode(lp, N1);
+      const T fac = fx * job.fs1[ky < 0 ? -ky : ky];
+      const int f = ret_fine(lp, n1, N1);
+      const int pos = rp + static_cast<int>(__brev(static_cast<unsigned>(f)) >> sh);
+      orow[static_cast<long long>(order_index(ky, N1, job.order_fft)) * job.st1] =
+          Cplx<T>{fre[pos] * fac, fim[pos] * fac};
+    }
   }
 }
 
